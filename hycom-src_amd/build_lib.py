@@ -1,0 +1,61 @@
+"""Builds the gfx950 hybgen library in-tree.
+
+    python hycom-src_amd/build_lib.py          # library only
+    python hycom-src_amd/build_lib.py --check  # + host build of the column code
+
+Output: hycom-src_amd/lib/libhycom_hybgen.so (HIP code objects for gfx950
+only).  Every HIP and host translation unit is compiled with
+-ffp-contract=off: the FP64 path must evaluate the same IEEE operations as
+the reference, and the host and device synthetic-input generators must
+agree bit for bit.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libhycom_hybgen.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}",
+]
+# no packed FP32 instructions in any device code
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
+          "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = ["hyb_kernels.hip", "hyb_capi.cpp"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-fopenmp", "-c",
+              os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *DEVICE, "-shared", "-fopenmp", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    print(LIB)
+    sys.exit(0)

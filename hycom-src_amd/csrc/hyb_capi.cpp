@@ -1,0 +1,454 @@
+// hyb_capi.cpp -- the C ABI of include/hycom_hybgen.h: parameter checks and
+// derivation (host), the copy-mode and device-resident hybgen calls, and
+// the synthetic-input generator entry points.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/hycom_hybgen.h"
+#include "hyb_kernels.h"
+#include "hyb_params_conv.h"
+
+using namespace hyb;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(e_ == hipErrorOutOfMemory ? HYB_ENOMEM : HYB_EDEVICE,        \
+                  "%s: %s", #expr, hipGetErrorString(e_));                     \
+  } while (0)
+
+struct Shape {
+  int64_t NI = 0, NJ = 0, NIJ = 0;
+  int K = 0, ntr = 0;
+  bool operator==(const Shape& o) const {
+    return NI == o.NI && NJ == o.NJ && K == o.K && ntr == o.ntr;
+  }
+};
+
+Shape shape_of(const hyb_params* P) {
+  Shape s;
+  s.NI = P->idm + 2 * P->nbdy;
+  s.NJ = P->jdm + 2 * P->nbdy;
+  s.NIJ = s.NI * s.NJ;
+  s.K = P->kdm;
+  s.ntr = P->ntracr;
+  return s;
+}
+
+int device_ready() {
+  static int state = 0;  // 0 unknown, 1 ok, <0 error code
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (state == 1) return 0;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1)
+    return fail(HYB_ENODEVICE, "no HIP device visible (%s)", hipGetErrorString(e));
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(HYB_ENODEVICE, "device %d is %s; this library is built for gfx950",
+                dev, prop.gcnArchName);
+  state = 1;
+  return 0;
+}
+
+int check_fields(const hyb_params* P, const hyb_fields* F, int raflag) {
+  if (!F) return fail(HYB_ENULL, "fields pointer is NULL");
+  const void* need[] = {F->dp, F->dpu, F->dpv, F->temp, F->saln, F->th3d, F->u, F->v,
+                        F->p, F->pu, F->pv, F->theta, F->depths, F->topiso,
+                        F->depthu, F->depthv, F->ubavg, F->vbavg, F->ip, F->iu, F->iv};
+  const char* names[] = {"dp", "dpu", "dpv", "temp", "saln", "th3d", "u", "v", "p", "pu",
+                         "pv", "theta", "depths", "topiso", "depthu", "depthv", "ubavg",
+                         "vbavg", "ip", "iu", "iv"};
+  for (size_t q = 0; q < sizeof need / sizeof need[0]; q++)
+    if (!need[q]) return fail(HYB_ENULL, "field %s is NULL", names[q]);
+  if (P->ntracr > 0 && !F->tracer) return fail(HYB_ENULL, "field tracer is NULL (ntracr>0)");
+  if (raflag && !F->dpo) return fail(HYB_ENULL, "field dpo is NULL (hybgen_raflag set)");
+  return 0;
+}
+
+// device mirrors for the copy-mode call
+struct Mirror {
+  Shape shape;
+  double *dp = nullptr, *dpo = nullptr, *dpu = nullptr, *dpv = nullptr;
+  double *temp = nullptr, *saln = nullptr, *th3d = nullptr, *u = nullptr, *v = nullptr;
+  double *tracer = nullptr, *p = nullptr, *pu = nullptr, *pv = nullptr, *theta = nullptr;
+  double *depths = nullptr, *topiso = nullptr, *depthu = nullptr, *depthv = nullptr;
+  double *ubavg = nullptr, *vbavg = nullptr;
+  int32_t *ip = nullptr, *iu = nullptr, *iv = nullptr;
+  hipStream_t stream = nullptr;
+  void release() {
+    void* ptrs[] = {dp, dpo, dpu, dpv, temp, saln, th3d, u, v, tracer, p, pu, pv, theta,
+                    depths, topiso, depthu, depthv, ubavg, vbavg, ip, iu, iv};
+    for (void* x : ptrs)
+      if (x) (void)hipFree(x);
+    Shape z;
+    *this = Mirror();
+    shape = z;
+  }
+};
+
+std::mutex g_mirror_mu;
+Mirror g_mirror;
+
+thread_local int32_t* g_rec_fixlay = nullptr;
+thread_local int32_t* g_rec_kp = nullptr;
+
+}  // namespace
+
+extern "C" {
+
+int hycom_hybgen_abi_version(void) { return HYB_ABI_VERSION; }
+
+const char* hycom_hybgen_strerror(int code) {
+  switch (code) {
+    case HYB_OK: return "ok";
+    case HYB_EINVAL: return "invalid parameter";
+    case HYB_EUNSUPPORTED: return "unsupported configuration";
+    case HYB_EDEVICE: return "HIP runtime error";
+    case HYB_ENOMEM: return "device out of memory";
+    case HYB_ENODEVICE: return "no usable gfx950 device";
+    case HYB_ENULL: return "required field pointer is NULL";
+    case HYB_ENONFINITE: return "non-finite input";
+    case HYB_EBUSY: return "no free slot or memory";
+    default: return "unknown error";
+  }
+}
+
+const char* hycom_hybgen_last_error(void) { return g_err; }
+
+int hycom_hybgen_check_params(const hyb_params* P) {
+  if (!P) return fail(HYB_ENULL, "params pointer is NULL");
+  if (P->kdm < 4 || P->kdm > HYB_KMAX)
+    return fail(HYB_EUNSUPPORTED, "kdm=%d outside 4..%d", P->kdm, HYB_KMAX);
+  if (P->nbdy < 1) return fail(HYB_EINVAL, "nbdy=%d must be >= 1", P->nbdy);
+  if (P->ii < 1 || P->jj < 1 || P->ii > P->idm || P->jj > P->jdm)
+    return fail(HYB_EINVAL, "active extent %dx%d outside array %dx%d", P->ii, P->jj,
+                P->idm, P->jdm);
+  if (P->ntracr < 0 || P->ntracr > HYB_TRMAX)
+    return fail(HYB_EINVAL, "ntracr=%d outside 0..%d", P->ntracr, HYB_TRMAX);
+  if (P->ntracr > 64) return fail(HYB_EUNSUPPORTED, "ntracr=%d > 64", P->ntracr);
+  // blkdat.F90:303-311 (nhybrd<=kdm); hybgen is only called when hybrid
+  if (P->nhybrd < 1 || P->nhybrd > P->kdm)
+    return fail(HYB_EINVAL, "nhybrd=%d outside 1..kdm", P->nhybrd);
+  if (!(P->qhybrlx > 0.0 && P->qhybrlx <= 1.0))  // blkdat.F90:946-954 hybrlx>=1
+    return fail(HYB_EINVAL, "qhybrlx=%g: hybrlx must be at least 1.0", P->qhybrlx);
+  if (!(P->hybthn >= 1.0))  // blkdat.F90:956-964
+    return fail(HYB_EINVAL, "hybthn=%g must be at least 1.0", P->hybthn);
+  if (!(P->hybthk >= 0.0 && P->hybthk <= 0.9))  // blkdat.F90:966-974
+    return fail(HYB_EINVAL, "hybthk=%g must be between 0.0 and 0.9", P->hybthk);
+  if (P->hybmap < 0 || P->hybmap > 3)  // blkdat.F90:987-996
+    return fail(HYB_EINVAL, "hybmap=%d must be 0..3", P->hybmap);
+  if (P->hybflg < 0 || P->hybflg > 2)  // blkdat.F90:997-1005
+    return fail(HYB_EINVAL, "hybflg=%d must be 0..2", P->hybflg);
+  if (P->hybflg != 0)  // blkdat.F90:1006-1012 (17-term EOS)
+    return fail(HYB_EINVAL, "hybflg must be 0 (T&S) for 17-term eqn. of state");
+  if ((P->isopcm != 0) != (P->hybiso > 0.0))  // blkdat.F90:986
+    return fail(HYB_EINVAL, "isopcm must equal hybiso>0");
+  for (int k = P->nhybrd; k < P->kdm; k++)  // blkdat.F90:271-281
+    if (P->dp0k[k] != 0.0) return fail(HYB_EINVAL, "dp0k must be zero for k>nhybrd");
+  if (P->mxlmy) return fail(HYB_EUNSUPPORTED, "mxlmy (MY2.5 q2/q2l) is not supported");
+  if (!P->isopcm && P->nhybrd < P->kdm)
+    return fail(HYB_EUNSUPPORTED,
+                "isopcm off with nhybrd<kdm carries PCM flags across columns of a row");
+  int it[3] = {P->itracr801, P->itracr802, P->itracr803};
+  for (int a = 0; a < 3; a++) {
+    if (it[a] < 0 || it[a] > P->ntracr)
+      return fail(HYB_EINVAL, "itracr(%d)=%d outside 0..ntracr", 801 + a, it[a]);
+    for (int b = 0; b < a; b++)
+      if (it[a] && it[a] == it[b])
+        return fail(HYB_EUNSUPPORTED, "itracr(%d) and itracr(%d) share a tracer", 801 + b,
+                    801 + a);
+    if (it[a] && !(P->baclin > 0.0))
+      return fail(HYB_EINVAL, "baclin must be > 0 with itracr(%d)", 801 + a);
+  }
+  if (P->precision != HYB_PREC_FP64 && P->precision != HYB_PREC_MIXED)
+    return fail(HYB_EINVAL, "precision=%d", P->precision);
+  if (P->precision == HYB_PREC_MIXED && P->itracr801 + P->itracr802 + P->itracr803 > 0)
+    return fail(HYB_EUNSUPPORTED, "diagnostic tracers need the FP64 path");
+  return 0;
+}
+
+int hycom_hybgen_derive_vgrid(hyb_params* P, double dp00, double dp00f, double dp00x,
+                              double ds00, double ds00f, double ds00x, double dp00i_m,
+                              double dx00, double dx00f, double dx00x, double hybrlx,
+                              double isotop_m, double* topiso_pa) {
+  if (!P) return fail(HYB_ENULL, "params pointer is NULL");
+  const int kk = P->kdm, nhybrd = P->nhybrd;
+  if (kk < 4 || kk > HYB_KMAX) return fail(HYB_EUNSUPPORTED, "kdm=%d", kk);
+  if (nhybrd < 1 || nhybrd > kk) return fail(HYB_EINVAL, "nhybrd=%d", nhybrd);
+  if (hybrlx < 1.0) return fail(HYB_EINVAL, "hybrlx must be at least 1.0");
+  if (isotop_m == 0.0) return fail(HYB_EINVAL, "isotop cannot be 0.0");  // blkdat.F90:403
+  if (dp00 < 0.0) return fail(HYB_EUNSUPPORTED, "dp0k input (dp00<0) not derived here");
+  // blkdat.F90:295-301
+  int nsigma = P->nsigma <= 1 ? 1 : P->nsigma;
+  if (nsigma > nhybrd) return fail(HYB_EINVAL, "maximum nsigma is nhybrd");
+  P->nsigma = nsigma;
+  // geopar.F90:449-482 maximum thickness
+  if (dx00 > 0.0) {
+    double x0 = kOnem * dx00, xx = kOnem * dx00x, f = 1.0;
+    P->dx0k[0] = x0;
+    for (int k = 2; k <= kk; k++) {
+      f = f * dx00f;
+      P->dx0k[k - 1] = dmin(x0 * f, xx);
+    }
+  } else {
+    for (int k = 1; k <= kk; k++) P->dx0k[k - 1] = kOnem * P->dx0k[k - 1];
+  }
+  // geopar.F90:520-600 deep and shallow z-level spacing
+  double d0 = kOnem * dp00, dx = kOnem * dp00x;
+  P->dp00i = kOnem * dp00i_m;
+  P->dp0k[0] = d0;
+  double f = 1.0;
+  for (int k = 2; k <= kk; k++) {
+    f = f * dp00f;
+    if (k <= nhybrd) {
+      P->dp0k[k - 1] = dp00f >= 1.0 ? dmin(d0 * f, dx) : dmax(d0 * f, dx);
+    } else {
+      P->dp0k[k - 1] = 0.0;
+    }
+  }
+  double s0 = kOnem * ds00, sx = kOnem * ds00x;
+  P->ds0k[0] = s0;
+  f = 1.0;
+  for (int k = 2; k <= nsigma; k++) {
+    f = f * ds00f;
+    P->ds0k[k - 1] = ds00f >= 1.0 ? dmin(s0 * f, sx) : dmax(s0 * f, sx);
+  }
+  // geopar.F90:609-630 terrain-following start and stop depths (nsigma>=1)
+  double dpns = 0.0, dsns = 0.0;
+  for (int k = 1; k <= nsigma; k++) {
+    dpns = dpns + P->dp0k[k - 1];
+    dsns = dsns + P->ds0k[k - 1];
+  }
+  for (int k = nsigma + 1; k <= kk; k++) P->ds0k[k - 1] = 0.0;
+  P->dpns = dpns * kQonem;
+  P->dsns = dsns * kQonem;
+  // blkdat.F90:954,986
+  P->qhybrlx = 1.0 / hybrlx;
+  P->isopcm = P->hybiso > 0.0;
+  if (topiso_pa) *topiso_pa = kOnem * isotop_m;  // mod_hycom.F90:1447
+  return 0;
+}
+
+int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev) {
+  g_rec_fixlay = fixlay_dev;
+  g_rec_kp = kp_dev;
+  return 0;
+}
+
+int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag,
+                        void* stream) {
+  int rc = hycom_hybgen_check_params(P);
+  if (rc) return rc;
+  if ((rc = check_fields(P, F, raflag))) return rc;
+  if (m < 1 || m > 2 || n < 1 || n > 2 || (raflag && m == n))
+    return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
+  if (P->precision != HYB_PREC_FP64)
+    return fail(HYB_EUNSUPPORTED, "mixed precision path not built yet");
+  if (P->check_finite) return fail(HYB_EUNSUPPORTED, "finite scan not built yet");
+  if ((rc = device_ready())) return rc;
+  KParams K = kparams_of(P);
+  View V = view_of(P, F);
+  V.rec_fixlay = g_rec_fixlay;
+  V.rec_kp = g_rec_kp;
+  HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream));
+  return 0;
+}
+
+int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag) {
+  int rc = hycom_hybgen_check_params(P);
+  if (rc) return rc;
+  if ((rc = check_fields(P, F, raflag))) return rc;
+  if ((rc = device_ready())) return rc;
+  std::lock_guard<std::mutex> lk(g_mirror_mu);
+  Mirror& M = g_mirror;
+  Shape s = shape_of(P);
+  const size_t n2 = (size_t)s.NIJ, n3 = n2 * s.K, n3p = n2 * (s.K + 1), n4 = 2 * n3;
+  const size_t n5 = n4 * (size_t)(s.ntr > 0 ? s.ntr : 1);
+  if (!(M.shape == s) || !M.dp) {
+    M.release();
+    HIPCHK(hipStreamCreate(&M.stream));
+    double** f4[] = {&M.dp, &M.dpo, &M.dpu, &M.dpv, &M.temp, &M.saln, &M.th3d, &M.u, &M.v};
+    for (double** x : f4) HIPCHK(hipMalloc(x, n4 * sizeof(double)));
+    HIPCHK(hipMalloc(&M.tracer, n5 * sizeof(double)));
+    HIPCHK(hipMalloc(&M.p, n3p * sizeof(double)));
+    HIPCHK(hipMalloc(&M.pu, n3p * sizeof(double)));
+    HIPCHK(hipMalloc(&M.pv, n3p * sizeof(double)));
+    HIPCHK(hipMalloc(&M.theta, n3 * sizeof(double)));
+    double** f2[] = {&M.depths, &M.topiso, &M.depthu, &M.depthv};
+    for (double** x : f2) HIPCHK(hipMalloc(x, n2 * sizeof(double)));
+    HIPCHK(hipMalloc(&M.ubavg, 3 * n2 * sizeof(double)));
+    HIPCHK(hipMalloc(&M.vbavg, 3 * n2 * sizeof(double)));
+    HIPCHK(hipMalloc(&M.ip, n2 * sizeof(int32_t)));
+    HIPCHK(hipMalloc(&M.iu, n2 * sizeof(int32_t)));
+    HIPCHK(hipMalloc(&M.iv, n2 * sizeof(int32_t)));
+    M.shape = s;
+  }
+  hipStream_t st = M.stream;
+  auto h2d = [&](void* d, const void* h, size_t bytes) {
+    return h ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+  };
+  auto d2h = [&](void* h, const void* d, size_t bytes) {
+    return h ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
+  };
+  const size_t b4 = n4 * sizeof(double);
+  HIPCHK(h2d(M.dp, F->dp, b4));
+  HIPCHK(h2d(M.dpo, F->dpo, b4));
+  HIPCHK(h2d(M.dpu, F->dpu, b4));
+  HIPCHK(h2d(M.dpv, F->dpv, b4));
+  HIPCHK(h2d(M.temp, F->temp, b4));
+  HIPCHK(h2d(M.saln, F->saln, b4));
+  HIPCHK(h2d(M.th3d, F->th3d, b4));
+  HIPCHK(h2d(M.u, F->u, b4));
+  HIPCHK(h2d(M.v, F->v, b4));
+  if (P->ntracr > 0) HIPCHK(h2d(M.tracer, F->tracer, n5 * sizeof(double)));
+  HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
+  HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
+  HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
+  HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
+  HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
+  HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
+  HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
+  HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
+  HIPCHK(h2d(M.ubavg, F->ubavg, 3 * n2 * sizeof(double)));
+  HIPCHK(h2d(M.vbavg, F->vbavg, 3 * n2 * sizeof(double)));
+  HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
+  HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
+  HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
+  hyb_fields D;
+  D.dp = M.dp; D.dpo = F->dpo ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
+  D.temp = M.temp; D.saln = M.saln; D.th3d = M.th3d; D.u = M.u; D.v = M.v;
+  D.tracer = P->ntracr > 0 ? M.tracer : nullptr;
+  D.p = M.p; D.pu = M.pu; D.pv = M.pv; D.theta = M.theta;
+  D.depths = M.depths; D.topiso = M.topiso; D.depthu = M.depthu; D.depthv = M.depthv;
+  D.ubavg = M.ubavg; D.vbavg = M.vbavg; D.ip = M.ip; D.iu = M.iu; D.iv = M.iv;
+  rc = hycom_hybgen_device(P, &D, m, n, raflag, st);
+  if (rc) return rc;
+  HIPCHK(d2h(F->dp, M.dp, b4));
+  HIPCHK(d2h(F->dpu, M.dpu, b4));
+  HIPCHK(d2h(F->dpv, M.dpv, b4));
+  HIPCHK(d2h(F->temp, M.temp, b4));
+  HIPCHK(d2h(F->saln, M.saln, b4));
+  HIPCHK(d2h(F->th3d, M.th3d, b4));
+  HIPCHK(d2h(F->u, M.u, b4));
+  HIPCHK(d2h(F->v, M.v, b4));
+  if (P->ntracr > 0) HIPCHK(d2h(F->tracer, M.tracer, n5 * sizeof(double)));
+  HIPCHK(d2h(F->p, M.p, n3p * sizeof(double)));
+  HIPCHK(d2h(F->pu, M.pu, n3p * sizeof(double)));
+  HIPCHK(d2h(F->pv, M.pv, n3p * sizeof(double)));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// synthetic inputs
+// ---------------------------------------------------------------------
+static double inv_normal_cdf(double pr) {
+  double lo = -10.0, hi = 10.0;
+  for (int it = 0; it < 200; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (0.5 * erfc(-mid / sqrt(2.0)) < pr) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+static int synth_domain(const hyb_params* P, const hyb_synth* S, SynthDomain* D) {
+  if (!P || !S) return fail(HYB_ENULL, "NULL argument");
+  if (P->kdm < 4 || P->kdm > HYB_KMAX) return fail(HYB_EUNSUPPORTED, "kdm=%d", P->kdm);
+  if (P->ntracr > 16) return fail(HYB_EUNSUPPORTED, "generator supports ntracr<=16");
+  if (S->itdm < 1 || S->jtdm < 1) return fail(HYB_EINVAL, "global extents");
+  memset(D, 0, sizeof *D);
+  D->seed = (uint64_t)S->seed;
+  D->itdm = S->itdm;
+  D->jtdm = S->jtdm;
+  D->i0 = S->i0;
+  D->j0 = S->j0;
+  D->periodic_i = !(P->nreg == 0 || P->nreg == 4);
+  D->periodic_j = P->nreg > 2;
+  D->K = P->kdm;
+  D->ntracr = P->ntracr;
+  D->extreme = S->extreme != 0;
+  D->topiso_pa = S->topiso_pa;
+  D->dmin = S->depth_min;
+  D->dmax = S->depth_max;
+  D->shelf = S->shelf_frac > 0.0 ? S->shelf_frac : 0.0;
+  D->sig_top = S->sigma_top;
+  D->sig_bot = S->sigma_bot;
+  D->uv_std = S->uv_std;
+  D->ts_noise = S->ts_noise;
+  D->thbase = P->thbase;
+  for (int t = 0; t < P->ntracr; t++) D->trcflg[t] = P->trcflg[t];
+  double ssq = 0.0;
+  for (int q = 0; q < kSynthModes; q++) {
+    uint64_t h = hash4((uint64_t)S->seed + 1, 1000 + q, 0, 0);
+    int kx = (int)(mix64(h) % 9) - 4;
+    int ky = (int)(mix64(h + 1) % 7) - 3;
+    if (kx == 0 && ky == 0) kx = 1;
+    double a = (0.5 + u01(mix64(h + 2))) / pow(1.0 + kx * kx + ky * ky, 0.75);
+    D->kx[q] = kx;
+    D->ky[q] = ky;
+    D->amp[q] = a;
+    D->ph[q] = kTwoPi * u01(mix64(h + 3));
+    ssq += a * a;
+  }
+  double sd = sqrt(0.5 * ssq);
+  for (int q = 0; q < kSynthModes; q++) D->amp[q] = D->amp[q] / sd;
+  D->thr = inv_normal_cdf(S->land_frac);
+  D->hmax = 2.5 - D->thr;
+  return 0;
+}
+
+int hycom_hybgen_synth_host(const hyb_params* P, const hyb_synth* S, const hyb_fields* F,
+                            int m, int n) {
+  SynthDomain D;
+  int rc = synth_domain(P, S, &D);
+  if (rc) return rc;
+  if ((rc = check_fields(P, F, 0))) return rc;
+  View V = view_of(P, F);
+  const int ii = P->ii, jj = P->jj;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = 0; j <= jj + 1; j++)
+    for (int i = 0; i <= ii + 1; i++) synth_point_pass1(D, V, ii, jj, i, j, m, n);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = 1; j <= jj; j++)
+    for (int i = 1; i <= ii; i++) synth_point_pass2(D, V, ii, jj, i, j, m, n);
+  return 0;
+}
+
+int hycom_hybgen_synth_device(const hyb_params* P, const hyb_synth* S, const hyb_fields* F,
+                              int m, int n, void* stream) {
+  SynthDomain D;
+  int rc = synth_domain(P, S, &D);
+  if (rc) return rc;
+  if ((rc = check_fields(P, F, 0))) return rc;
+  if ((rc = device_ready())) return rc;
+  View V = view_of(P, F);
+  HIPCHK(launch_synth(D, V, P->ii, P->jj, m, n, (hipStream_t)stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,299 @@
+"""ctypes mirror of include/hycom_hybgen.h plus field-array helpers.
+
+Host fields are numpy arrays in HYCOM's mod_cb_arrays layout (Fortran order,
+halo included); device fields are flat torch tensors of the same sizes
+(torch is used for device memory and streams only).  The compute path is the
+HIP library; loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhycom_hybgen.so")
+
+KMAX = 64
+TRMAX = 99
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+HYB_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EUNSUPPORTED", -3: "EDEVICE", -4: "ENOMEM",
+          -5: "ENODEVICE", -6: "ENULL", -7: "ENONFINITE", -8: "EBUSY"}
+
+_INT_FIELDS = ["idm", "jdm", "ii", "jj", "nbdy", "kdm", "ntracr", "nhybrd",
+               "nsigma", "hybmap", "hybflg", "nreg", "isopcm", "mxlmy",
+               "ldenfix", "itracr801", "itracr802", "itracr803", "precision",
+               "check_finite"]
+_DBL_FIELDS = ["thbase", "qhybrlx", "hybiso", "hybthn", "hybthk", "thkbot",
+               "dp00i", "dpns", "dsns", "baclin", "ra2fac", "vland"]
+
+
+class HybParams(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int32) for n in _INT_FIELDS]
+                + [("trcflg", ctypes.c_int32 * TRMAX)]
+                + [(n, ctypes.c_double) for n in _DBL_FIELDS]
+                + [("dp0k", ctypes.c_double * KMAX),
+                   ("ds0k", ctypes.c_double * KMAX),
+                   ("dx0k", ctypes.c_double * KMAX)])
+
+
+FIELD_NAMES = ["dp", "dpo", "dpu", "dpv", "temp", "saln", "th3d", "u", "v",
+               "tracer", "p", "pu", "pv", "theta", "depths", "topiso",
+               "depthu", "depthv", "ubavg", "vbavg", "ip", "iu", "iv"]
+
+
+class HybFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in FIELD_NAMES]
+
+
+class HybSynth(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_int64),
+                ("itdm", ctypes.c_int32), ("jtdm", ctypes.c_int32),
+                ("i0", ctypes.c_int32), ("j0", ctypes.c_int32),
+                ("depth_min", ctypes.c_double), ("depth_max", ctypes.c_double),
+                ("land_frac", ctypes.c_double), ("shelf_frac", ctypes.c_double),
+                ("sigma_top", ctypes.c_double), ("sigma_bot", ctypes.c_double),
+                ("uv_std", ctypes.c_double), ("ts_noise", ctypes.c_double),
+                ("topiso_pa", ctypes.c_double),
+                ("extreme", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# every symbol include/hycom_hybgen.h declares, with its ctypes signature
+_P = ctypes.POINTER
+SIGNATURES = {
+    "hycom_hybgen_abi_version": (ctypes.c_int, []),
+    "hycom_hybgen_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "hycom_hybgen_last_error": (ctypes.c_char_p, []),
+    "hycom_hybgen_check_params": (ctypes.c_int, [_P(HybParams)]),
+    "hycom_hybgen_derive_vgrid": (ctypes.c_int, [_P(HybParams)] + [ctypes.c_double] * 12
+                                  + [_P(ctypes.c_double)]),
+    "hycom_hybgen": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int]),
+    "hycom_hybgen_device": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "hycom_hybgen_synth_host": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
+                                               ctypes.c_int, ctypes.c_int]),
+    "hycom_hybgen_synth_device": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with "
+                           "`python hycom-src_amd/build_lib.py` (no fallback path exists)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HybError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hybgen error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != HYB_OK:
+        raise HybError(rc, load().hycom_hybgen_last_error().decode())
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# parameter presets for the documented workloads
+# ---------------------------------------------------------------------------
+WORKLOADS = {
+    # 150x100x32, 2 tracers, sigma-2 28.10 -> 37.00, PPM
+    "cfg1": dict(idm=150, jdm=100, kdm=32, ntracr=2, trcflg=[2, 0], hybmap=2,
+                 sigma=(28.10, 37.00), depth=(50.0, 5500.0), land=0.30, shelf=0.10,
+                 dp00=3.0, dp00f=1.125, dp00x=12.0, ds00=1.0, ds00f=1.125, ds00x=12.0,
+                 nsigma=8, nreg=0),
+    # GLBb 1/12 deg shape: 4500x3298x41, 4 tracers, sigma-2 17.00 -> 37.48
+    "glbb": dict(idm=4500, jdm=3298, kdm=41, ntracr=4, trcflg=[0, 0, 0, 0], hybmap=2,
+                 sigma=(17.00, 37.48), depth=(5.0, 6500.0), land=0.35, shelf=0.10,
+                 dp00=1.0, dp00f=1.18, dp00x=450.0, ds00=1.0, ds00f=1.18, ds00x=450.0,
+                 nsigma=14, nreg=1),
+}
+
+
+def make_params(kdm=32, idm=150, jdm=100, ntracr=2, trcflg=(2, 0), hybmap=2,
+                nhybrd=None, nsigma=8, nreg=0, dp00=3.0, dp00f=1.125, dp00x=12.0,
+                ds00=1.0, ds00f=1.125, ds00x=12.0, dp00i=1.0, dx00=9999.0,
+                dx00f=1.0, dx00x=9999.0, hybrlx=8.0, hybiso=0.01, hybthn=1.0,
+                hybthk=0.0, thkbot=10.0, isotop=6.0, thbase=34.0, ldenfix=0,
+                nbdy=6, ii=None, jj=None, precision=PREC_FP64, baclin=300.0,
+                ra2fac=0.125, **_unused):
+    """hyb_params from blkdat.input-style values (geopar derivation in C)."""
+    P = HybParams()
+    P.idm, P.jdm = idm, jdm
+    P.ii = idm if ii is None else ii
+    P.jj = jdm if jj is None else jj
+    P.nbdy = nbdy
+    P.kdm = kdm
+    P.ntracr = ntracr
+    for t, f in enumerate(trcflg[:ntracr]):
+        P.trcflg[t] = f
+    P.nhybrd = kdm if nhybrd is None else nhybrd
+    P.nsigma = nsigma
+    P.hybmap = hybmap
+    P.hybflg = 0
+    P.nreg = nreg
+    P.ldenfix = ldenfix
+    P.precision = precision
+    P.thbase = thbase
+    P.hybiso = hybiso
+    P.hybthn = hybthn
+    P.hybthk = hybthk
+    P.thkbot = thkbot
+    P.baclin = baclin
+    P.ra2fac = ra2fac
+    P.vland = 0.0
+    topiso = ctypes.c_double(0.0)
+    check(load().hycom_hybgen_derive_vgrid(
+        ctypes.byref(P), dp00, dp00f, dp00x, ds00, ds00f, ds00x, dp00i,
+        dx00, dx00f, dx00x, hybrlx, isotop, ctypes.byref(topiso)))
+    P._topiso_pa = topiso.value
+    return P
+
+
+def workload_params(name, **over):
+    w = dict(WORKLOADS[name])
+    w.update(over)
+    P = make_params(**w)
+    return P, w
+
+
+def make_synth(P, w, seed=100, i0=0, j0=0, itdm=None, jtdm=None, extreme=0):
+    S = HybSynth()
+    S.seed = seed
+    S.itdm = P.idm if itdm is None else itdm
+    S.jtdm = P.jdm if jtdm is None else jtdm
+    S.i0, S.j0 = i0, j0
+    S.depth_min, S.depth_max = w["depth"]
+    S.land_frac = w["land"]
+    S.shelf_frac = w.get("shelf", 0.1)
+    S.sigma_top, S.sigma_bot = w["sigma"]
+    S.uv_std = 0.1
+    S.ts_noise = 0.05
+    S.topiso_pa = P._topiso_pa
+    S.extreme = extreme
+    return S
+
+
+# ---------------------------------------------------------------------------
+# field storage
+# ---------------------------------------------------------------------------
+def field_shapes(P):
+    NI = P.idm + 2 * P.nbdy
+    NJ = P.jdm + 2 * P.nbdy
+    K = P.kdm
+    ntr = max(P.ntracr, 1)
+    s4 = (NI, NJ, K, 2)
+    return {
+        "dp": s4, "dpo": s4, "dpu": s4, "dpv": s4, "temp": s4, "saln": s4,
+        "th3d": s4, "u": s4, "v": s4, "tracer": (NI, NJ, K, 2, ntr),
+        "p": (NI, NJ, K + 1), "pu": (NI, NJ, K + 1), "pv": (NI, NJ, K + 1),
+        "theta": (NI, NJ, K), "depths": (NI, NJ), "topiso": (NI, NJ),
+        "depthu": (NI, NJ), "depthv": (NI, NJ), "ubavg": (NI, NJ, 3),
+        "vbavg": (NI, NJ, 3), "ip": (NI, NJ), "iu": (NI, NJ), "iv": (NI, NJ),
+    }
+
+
+INT_ARRAYS = ("ip", "iu", "iv")
+OUTPUTS = ("dp", "dpu", "dpv", "temp", "saln", "th3d", "u", "v", "tracer",
+           "p", "pu", "pv")
+
+
+class HostDomain:
+    """All hybgen fields of one domain as Fortran-ordered numpy arrays."""
+
+    def __init__(self, P, arrays=None):
+        self.P = P
+        shapes = field_shapes(P)
+        if arrays is None:
+            arrays = {}
+            for n, s in shapes.items():
+                dt = np.int32 if n in INT_ARRAYS else np.float64
+                arrays[n] = np.zeros(s, dtype=dt, order="F")
+        self.a = arrays
+
+    def copy(self):
+        return HostDomain(self.P, {n: v.copy(order="F") for n, v in self.a.items()})
+
+    def fields(self):
+        F = HybFields()
+        for n in FIELD_NAMES:
+            setattr(F, n, self.a[n].ctypes.data)
+        if self.P.ntracr == 0:
+            F.tracer = None
+        return F
+
+    def synth(self, S, m=1, n=2):
+        F = self.fields()
+        check(load().hycom_hybgen_synth_host(ctypes.byref(self.P), ctypes.byref(S),
+                                             ctypes.byref(F), m, n))
+        return self
+
+    def hybgen(self, m=1, n=2, raflag=0):
+        """Copy-mode drop-in call through the C ABI (needs a GPU)."""
+        F = self.fields()
+        check(load().hycom_hybgen(ctypes.byref(self.P), ctypes.byref(F), m, n, raflag))
+        return self
+
+
+class DeviceDomain:
+    """All hybgen fields of one domain as flat device tensors."""
+
+    def __init__(self, P, device="cuda"):
+        import torch
+        self.P = P
+        self.t = {}
+        for n, s in field_shapes(P).items():
+            dt = torch.int32 if n in INT_ARRAYS else torch.float64
+            self.t[n] = torch.zeros(int(np.prod(s)), dtype=dt, device=device)
+
+    def fields(self):
+        F = HybFields()
+        for n in FIELD_NAMES:
+            setattr(F, n, self.t[n].data_ptr())
+        if self.P.ntracr == 0:
+            F.tracer = None
+        return F
+
+    def upload(self, host):
+        import torch
+        for n, v in host.a.items():
+            self.t[n].copy_(torch.from_numpy(np.ascontiguousarray(v.reshape(-1, order="F"))))
+        return self
+
+    def download(self, names=None):
+        shapes = field_shapes(self.P)
+        out = {}
+        for n in (names or shapes.keys()):
+            out[n] = self.t[n].cpu().numpy().reshape(shapes[n], order="F")
+        return HostDomain(self.P, out) if names is None else out
+
+    def synth(self, S, m=1, n=2, stream=None):
+        F = self.fields()
+        check(load().hycom_hybgen_synth_device(ctypes.byref(self.P), ctypes.byref(S),
+                                               ctypes.byref(F), m, n, stream))
+        return self
+
+    def hybgen(self, m=1, n=2, raflag=0, stream=None):
+        F = self.fields()
+        check(load().hycom_hybgen_device(ctypes.byref(self.P), ctypes.byref(F), m, n,
+                                         raflag, stream))
+        return self

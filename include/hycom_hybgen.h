@@ -1,0 +1,213 @@
+/*
+ * hycom_hybgen.h -- C ABI of the MI355X (gfx950) hybrid-coordinate generator.
+ *
+ * This library replaces HYCOM's
+ *     subroutine hybgen(m,n,hybgen_raflag)            (hybgen.F90:14-306)
+ * and everything below it on the call path (hybgenaj hybgen.F90:308-1756,
+ * hybgenbj :1758-1907, hybgencj :1909-2274, the PCM/PLM/PPM/WENO remappers
+ * :2276-3107, dpudpv dpudpv.F90:14-107 and the p halo of xctilr,
+ * mod_xc_sm.h:1337-1429).  The reference has no array arguments: every field
+ * is a module array of mod_cb_arrays (mod_cb_arrays.F90:13-33, 68-81,
+ * 279-290).  Here the same arrays are passed as plain pointers, each one in
+ * the unchanged Fortran layout
+ *
+ *     x(1-nbdy:idm+nbdy, 1-nbdy:jdm+nbdy, 1:kdm, 1:2 [, 1:ntracr])
+ *
+ * column-major, i fastest; element (i,j,k,l) sits at
+ *     (i-1+nbdy) + NI*((j-1+nbdy) + NJ*((k-1) + kdm*(l-1)))
+ * with NI = idm+2*nbdy, NJ = jdm+2*nbdy.  p, pu, pv have kdm+1 levels and no
+ * time level; theta has kdm levels and no time level; ubavg/vbavg have 3
+ * time levels (mod_cb_arrays.F90:1159-1160).
+ *
+ * Error convention: 0 on success, a negative HYB_E* code otherwise (the
+ * reference calls xcstop instead, blkdat.F90:946-1012, dpudpv.F90:47-55).
+ * hycom_hybgen_strerror() names the code; hycom_hybgen_last_error() gives a
+ * sentence about the last failure on the calling thread.
+ */
+#ifndef HYCOM_HYBGEN_H
+#define HYCOM_HYBGEN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYB_ABI_VERSION 1
+
+/* Largest supported layer count and tracer count.  kdm <= 64 lets one
+ * 64-lane wavefront hold a whole column level-by-level; mxtrcr is the
+ * reference's own tracer limit (mod_dimensions.F90:26). */
+#define HYB_KMAX   64
+#define HYB_TRMAX  99
+
+/* return codes */
+#define HYB_OK                0
+#define HYB_EINVAL           -1  /* parameter outside what blkdat.F90 accepts   */
+#define HYB_EUNSUPPORTED     -2  /* valid for the reference, not built here     */
+#define HYB_EDEVICE          -3  /* HIP runtime error                           */
+#define HYB_ENOMEM           -4  /* device allocation failed                    */
+#define HYB_ENODEVICE        -5  /* no usable gfx950 device                     */
+#define HYB_ENULL            -6  /* a required field pointer is NULL            */
+#define HYB_ENONFINITE       -7  /* NaN/Inf found by the optional input scan    */
+#define HYB_EBUSY            -8  /* serving: no slot or memory free right now   */
+
+/* arithmetic modes */
+#define HYB_PREC_FP64   0  /* FP64 throughout, bit-faithful to the reference   */
+#define HYB_PREC_MIXED  1  /* FP32 reconstruction/remap, FP64 interfaces and
+                              compensated overlap sums                          */
+
+/*
+ * Run-time parameters of one hybgen call.  Every member is the value of the
+ * reference variable of the same name after blkdat/geopar set-up, in the
+ * reference's units (pressure in Pa, i.e. metres times onem), so that a
+ * Fortran shim copies them one-for-one:
+ *   mod_dimensions.F90:45,49,68 (idm, jdm, kdm, ii, jj), :33 (nbdy)
+ *   mod_cb_arrays.F90:279-290 (dx0k, dp0k, ds0k, dpns, dsns),
+ *   :318-366 (isopcm, mxlmy), :807-822 (thbase, qhybrlx, hybiso, hybthn,
+ *   hybthk, thkbot, dp00i, ra2fac, baclin), :824-833 (nhybrd, nsigma,
+ *   hybmap, hybflg, ntracr, trcflg, itracr)
+ *   mod_xc.F90 (nreg, vland)
+ *   hybgen.F90:319-325 (the HYBGEN_DENFIX cpp macro becomes ldenfix)
+ */
+typedef struct hyb_params {
+  int32_t idm, jdm;      /* array extents without halo                       */
+  int32_t ii, jj;        /* active extents, 1..ii x 1..jj (ii<=idm, jj<=jdm) */
+  int32_t nbdy;          /* halo width (6 in HYCOM)                          */
+  int32_t kdm;           /* number of layers, kk == kdm                      */
+  int32_t ntracr;        /* tracers in the trailing tracer index             */
+  int32_t nhybrd;        /* number of hybrid layers                          */
+  int32_t nsigma;        /* number of sigma levels (informational here)      */
+  int32_t hybmap;        /* 0 PCM, 1 PLM, 2 PPM, 3 WENO-like                 */
+  int32_t hybflg;        /* 0 T&S (the only value allowed with EOS_17T)      */
+  int32_t nreg;          /* region type: 0,4 closed in i; else periodic in i;
+                            <=2 closed in j, else periodic in j              */
+  int32_t isopcm;        /* logical: hybiso > 0                              */
+  int32_t mxlmy;         /* logical: MY2.5 q2/q2l fields (must be 0 here)     */
+  int32_t ldenfix;       /* logical: the HYBGEN_DENFIX variant of fixlay     */
+  int32_t itracr801;     /* tracer index or 0: dp change diagnostic          */
+  int32_t itracr802;     /* tracer index or 0: temp change diagnostic        */
+  int32_t itracr803;     /* tracer index or 0: saln change diagnostic        */
+  int32_t precision;     /* HYB_PREC_FP64 or HYB_PREC_MIXED                  */
+  int32_t check_finite;  /* logical: scan sea-point inputs for NaN/Inf first */
+  int32_t trcflg[HYB_TRMAX];
+  double  thbase, qhybrlx, hybiso, hybthn, hybthk, thkbot, dp00i;
+  double  dpns, dsns;    /* metres (geopar.F90:629-630)                      */
+  double  baclin;        /* baroclinic time step, s (itracr 801-803 only)    */
+  double  ra2fac;        /* Robert-Asselin weight (hybgen_raflag only)       */
+  double  vland;         /* halo fill value for closed boundaries            */
+  double  dp0k[HYB_KMAX], ds0k[HYB_KMAX], dx0k[HYB_KMAX];  /* Pa             */
+} hyb_params;
+
+/*
+ * Field pointers, each in the layout described at the top of this file.
+ * In hycom_hybgen() they are host pointers; in hycom_hybgen_device() and
+ * the batched/serving calls they are device pointers.  Fields the call does
+ * not touch may be NULL (dpo unless raflag, tracer when ntracr==0).
+ */
+typedef struct hyb_fields {
+  double *dp, *dpo, *dpu, *dpv;        /* (NI,NJ,kdm,2)                     */
+  double *temp, *saln, *th3d;          /* (NI,NJ,kdm,2)                     */
+  double *u, *v;                       /* (NI,NJ,kdm,2)                     */
+  double *tracer;                      /* (NI,NJ,kdm,2,ntracr)              */
+  double *p, *pu, *pv;                 /* (NI,NJ,kdm+1)                     */
+  const double *theta;                 /* (NI,NJ,kdm)                       */
+  const double *depths, *topiso;       /* (NI,NJ); depths in m, topiso Pa   */
+  const double *depthu, *depthv;       /* (NI,NJ), Pa                       */
+  const double *ubavg, *vbavg;         /* (NI,NJ,3)                         */
+  const int32_t *ip, *iu, *iv;         /* (NI,NJ) land/sea masks            */
+} hyb_fields;
+
+/* ---------------------------------------------------------------------- */
+/* Library information                                                    */
+/* ---------------------------------------------------------------------- */
+int         hycom_hybgen_abi_version(void);
+const char* hycom_hybgen_strerror(int code);
+const char* hycom_hybgen_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Parameter set-up (host only, no device needed)                         */
+/* ---------------------------------------------------------------------- */
+
+/* Checks the parameter pack the way blkdat.F90:946-1012 checks the
+ * blkdat.input values (hybrlx>=1 i.e. 0<qhybrlx<=1, hybthn>=1,
+ * 0<=hybthk<=0.9, 0<=hybmap<=3, hybflg==0 for the 17-term EOS) plus the
+ * limits of this build (kdm, ntracr, mxlmy). */
+int hycom_hybgen_check_params(const hyb_params* P);
+
+/* Derives dp0k, ds0k, dx0k, dpns, dsns exactly as geopar.F90:449-630 does
+ * from the blkdat.input values (dp00* / ds00* / dx00* in metres, or
+ * dx00 <= 0 for "dx0k given", in which case dx0k[] is taken as metres);
+ * also sets qhybrlx = 1/hybrlx and isopcm = hybiso > 0
+ * (blkdat.F90:954, 987).  isopyc (MICOM mode) is not supported. */
+int hycom_hybgen_derive_vgrid(hyb_params* P,
+                              double dp00, double dp00f, double dp00x,
+                              double ds00, double ds00f, double ds00x,
+                              double dp00i_m, double dx00, double dx00f,
+                              double dx00x, double hybrlx, double isotop_m,
+                              double* topiso_pa);
+
+/* ---------------------------------------------------------------------- */
+/* The hybgen call                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Drop-in for  call hybgen(m,n,hybgen_raflag)  (mod_hycom.F90:2650,
+ * inicon.F90:544,559): host arrays in, host arrays out.  Copies the fields
+ * the call reads to device mirrors the library keeps between calls, runs
+ * the device path, copies the fields it writes back, and returns when the
+ * host arrays hold the result.  m, n are the 1-based time levels. */
+int hycom_hybgen(const hyb_params* P, const hyb_fields* F,
+                 int m, int n, int raflag);
+
+/* Same call on device-resident fields (device pointers in F), ordered on
+ * the given HIP stream (hipStream_t passed as void*, NULL = default
+ * stream).  Returns after enqueueing; the caller synchronises. */
+int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
+                        int m, int n, int raflag, void* stream);
+
+/* Per-column decisions of the last hycom_hybgen_device() call on this
+ * thread's library context, for decision-parity checks: fixlay and kp of
+ * every p-point (idm*jdm int32 each, i fastest, 0 on land).  Device
+ * pointers; NULL disables recording (the default). */
+int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev);
+
+/* ---------------------------------------------------------------------- */
+/* Synthetic inputs (workload generator; no reference counterpart)         */
+/* ---------------------------------------------------------------------- */
+
+/* Description of one synthetic domain.  The generator is a pure function
+ * of (seed, global i, global j), evaluated with the same IEEE operations on
+ * the host and on the device, so a window generated on the host is
+ * bit-identical to the same window generated on the device. */
+typedef struct hyb_synth {
+  int64_t seed;
+  int32_t itdm, jtdm;      /* global extents the window is part of         */
+  int32_t i0, j0;          /* window offset in the global grid             */
+  double  depth_min, depth_max;   /* m                                     */
+  double  land_frac;       /* target land fraction                          */
+  double  shelf_frac;      /* fraction of sea made shallow shelf (0..1)     */
+  double  sigma_top, sigma_bot;   /* sigma-2 targets, evenly spaced         */
+  double  uv_std;          /* m/s                                           */
+  double  ts_noise;        /* psu, salinity noise std                       */
+  double  topiso_pa;       /* value written to topiso (onem*isotop), Pa     */
+  int32_t extreme;         /* 1: 40 C / 40 psu fronts and 1e-3 m layers     */
+  int32_t reserved;
+} hyb_synth;
+
+/* Fills every field of F (host pointers) and theta/masks/depths for the
+ * window described by S and P, in the layout above, time level n, and
+ * copies level n to level m (m != n).  Parameters in P (dp0k, thbase...)
+ * must already be set. */
+int hycom_hybgen_synth_host(const hyb_params* P, const hyb_synth* S,
+                            const hyb_fields* F, int m, int n);
+
+/* Same on device pointers, ordered on `stream`. */
+int hycom_hybgen_synth_device(const hyb_params* P, const hyb_synth* S,
+                              const hyb_fields* F, int m, int n,
+                              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYCOM_HYBGEN_H */

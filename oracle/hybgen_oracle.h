@@ -1,0 +1,48 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * A plain-C, FP64, single-threaded-per-row restatement of HYCOM's hybgen
+ * call path, used only by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  Nothing in the shipped
+ * library links or calls it.  Each function cites the reference lines it
+ * restates (paths relative to the HYCOM-src tree).
+ *
+ * Pinning: every function here is compared bit-for-bit against the
+ * reference Fortran compiled from its own sources by oracle/ref/Makefile
+ * (outputs in oracle/_ref/, see tests/test_oracle_vs_ref.py), and against
+ * committed golden vectors written by that build (tests/golden/).
+ */
+#ifndef HYBGEN_ORACLE_H
+#define HYBGEN_ORACLE_H
+
+#include "../include/hycom_hybgen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Full hybgen(m,n,raflag) on host arrays, in place (hybgen.F90:14-306).
+ * Returns 0, or HYB_EINVAL/HYB_EUNSUPPORTED for parameter packs the
+ * restatement does not cover. */
+int oracle_hybgen(const hyb_params* P, const hyb_fields* F,
+                  int m, int n, int raflag);
+
+/* Same, with the j loop split across nthreads OpenMP threads the way the
+ * reference's !$OMP PARALLEL DO over j does (hybgen.F90:89-95). */
+int oracle_hybgen_omp(const hyb_params* P, const hyb_fields* F,
+                      int m, int n, int raflag, int nthreads);
+
+/* sigma-2, 17-term EOS (stmt_fns.h:505-511). */
+double oracle_sig(double t, double s);
+
+/* Single-column remappers on caller arrays, 1 field (ks=1), for unit
+ * tests: method 0 PCM, 1 PLM, 2 PPM, 3 WENO.  si[k], pi[k+1], po[k+1],
+ * lcm[k] (0/1), so[k] are 0-based C arrays of the Fortran 1-based ones. */
+void oracle_remap_column(int method, int kk, const double* si,
+                         const double* pi, const double* po,
+                         const int* lcm, double thin, double* so);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

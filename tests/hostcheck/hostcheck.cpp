@@ -1,0 +1,75 @@
+// Host build of the device column code (hycom-src_amd/csrc/hyb_column.h),
+// run in the same phase order as the kernel launcher, so the CPU suite can
+// check the kernels' own arithmetic against the oracle without a GPU.
+// Test infrastructure only: nothing in the library calls this.
+#include "hyb_column.h"
+#include "hyb_params_conv.h"
+
+using namespace hyb;
+
+static uint64_t row_lcm(const KParams& P) {
+  uint64_t m = 0;
+  if (!P.isopcm)
+    for (int k = P.nhybrd + 1; k <= P.kk; k++) m |= 1ull << (k - 1);
+  return m;
+}
+
+template <int KC>
+static void run(const KParams& P, const View& V, int m, int n, int raflag) {
+  const int ii = P.ii, jj = P.jj, kk = P.kk;
+  const int itr[3] = {P.itr801, P.itr802, P.itr803};
+  double* src[3] = {V.dp, V.temp, V.saln};
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          for (int k = 1; k <= kk; k++)
+            V.tracer[ij_of(V, i, j) + levt(V, k, n, itr[d])] = src[d][ij_of(V, i, j) + lev4(V, k, n)];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 1; j <= jj; j++)
+    for (int i = 1; i <= ii; i++)
+      if (V.ip[ij_of(V, i, j)]) {
+        uint64_t l = row_lcm(P);
+        column_a<KC>(P, V, i, j, n, l);
+      }
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0) {
+      double q = d == 0 ? P.diag_q801 : P.diag_q80x;
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)])
+            for (int k = 1; k <= kk; k++) {
+              int64_t a = ij_of(V, i, j) + levt(V, k, n, itr[d]);
+              V.tracer[a] = q * (src[d][ij_of(V, i, j) + lev4(V, k, n)] - V.tracer[a]);
+            }
+    }
+  for (int pass = 0; pass <= (raflag ? 1 : 0); pass++) {
+    int nl = pass == 0 ? n : m;
+    if (pass == 1) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)]) column_ra<KC>(P, V, i, j, m, n, row_lcm(P));
+    }
+    for (int k = 2; k <= kk + 1; k++)
+      for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
+    for (int k = 2; k <= kk + 1; k++)
+      for (int j = 0; j <= jj + 1; j++) p_halo_icol(P, V, j, k);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 1; j <= jj; j++)
+      for (int isv = 0; isv <= 1; isv++)
+        for (int i = 1; i <= ii; i++)
+          if ((isv ? V.iv : V.iu)[ij_of(V, i, j)]) point_c<KC>(P, V, i, j, nl, isv != 0);
+  }
+}
+
+extern "C" int hostcheck_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n,
+                                int raflag) {
+  KParams K = kparams_of(P);
+  View V = view_of(P, F);
+  if (K.kk <= 24) run<24>(K, V, m, n, raflag);
+  else if (K.kk <= 32) run<32>(K, V, m, n, raflag);
+  else if (K.kk <= 48) run<48>(K, V, m, n, raflag);
+  else run<64>(K, V, m, n, raflag);
+  return 0;
+}

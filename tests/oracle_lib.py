@@ -1,0 +1,94 @@
+"""Loaders for the checkers: the C restatement (oracle/liboracle_hybgen.so)
+and the flang build of the reference hybgen (oracle/_ref/*.so).  Test
+infrastructure only."""
+import ctypes
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "hycom-src_amd"))
+import hybgen_abi as abi  # noqa: E402
+
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_hybgen.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_P = ctypes.POINTER
+_oracle = None
+_refs = {}
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.oracle_hybgen.argtypes = [_P(abi.HybParams), _P(abi.HybFields),
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.oracle_hybgen_omp.argtypes = [_P(abi.HybParams), _P(abi.HybFields),
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int]
+        lib.oracle_sig.restype = ctypes.c_double
+        lib.oracle_sig.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.oracle_remap_column.argtypes = [
+            ctypes.c_int, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
+            _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_double, _P(ctypes.c_double)]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available(denfix=False):
+    return os.path.exists(ref_path(denfix))
+
+
+def ref_path(denfix=False):
+    return os.path.join(REF_DIR, "libhybgen_ref_denfix.so" if denfix else "libhybgen_ref.so")
+
+
+def ref(denfix=False):
+    if denfix not in _refs:
+        lib = ctypes.CDLL(ref_path(denfix))
+        lib.href_hybgen.restype = ctypes.c_int
+        lib.href_hybgen.argtypes = [_P(abi.HybParams), _P(abi.HybFields), ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, _P(ctypes.c_double)]
+        lib.href_sig.restype = ctypes.c_double
+        lib.href_sig.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.href_consts.argtypes = [_P(ctypes.c_double)]
+        lib.href_remap.argtypes = [
+            ctypes.c_int, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
+            _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_double, _P(ctypes.c_double)]
+        _refs[denfix] = lib
+    return _refs[denfix]
+
+
+def run_oracle(dom, m=1, n=2, raflag=0, threads=1):
+    F = dom.fields()
+    rc = oracle().oracle_hybgen_omp(ctypes.byref(dom.P), ctypes.byref(F), m, n, raflag,
+                                    threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_hybgen rc={rc}")
+    return dom
+
+
+def run_ref(dom, m=1, n=2, raflag=0):
+    F = dom.fields()
+    secs = ctypes.c_double(0.0)
+    rc = ref(bool(dom.P.ldenfix)).href_hybgen(ctypes.byref(dom.P), ctypes.byref(F), m, n,
+                                              raflag, ctypes.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"href_hybgen rc={rc}")
+    return secs.value
+
+
+def diff_domains(a, b, names=abi.OUTPUTS):
+    """Names of arrays that differ bitwise (NaN-aware), with counts."""
+    import numpy as np
+    out = {}
+    for n in names:
+        x, y = a.a[n], b.a[n]
+        if x.dtype.kind == "f":
+            bad = ~((x == y) | (np.isnan(x) & np.isnan(y)))
+        else:
+            bad = x != y
+        c = int(bad.sum())
+        if c:
+            out[n] = c
+    return out
