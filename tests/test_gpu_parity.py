@@ -1,0 +1,104 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, bitwise.
+
+Every case prints its name, shape and seed before touching the GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import hybgen_abi as abi
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **over):
+    print(f"\n[case] {name} {over}", flush=True)
+    P, w = abi.workload_params(over.pop("workload", "cfg1"), **over)
+    return P, w
+
+
+def _oracle_out(d0, m=1, n=2, raflag=0):
+    a = d0.copy()
+    ol.run_oracle(a, m=m, n=n, raflag=raflag, threads=8)
+    return a
+
+
+def _assert_same(a, b, what):
+    df = ol.diff_domains(a, b)
+    assert not df, f"{what}: arrays differ bitwise: {df}"
+
+
+@pytest.mark.parametrize("hybmap", [0, 1, 2, 3])
+def test_device_path_matches_oracle_cfg1(gpu, hybmap):
+    P, w = _case("cfg1-device", hybmap=hybmap)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=100))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    import torch
+    torch.cuda.synchronize()
+    got = dd.download()
+    _assert_same(want, got, f"hybmap={hybmap}")
+
+
+@pytest.mark.parametrize("variant", [
+    dict(ldenfix=1), dict(hybthk=0.5, hybthn=5.0), dict(nsigma=0, hybrlx=1.0),
+    dict(nreg=1, dx00=30.0, dx00x=300.0, dx00f=1.1), dict(nreg=3, hybmap=3),
+    dict(kdm=22, ntracr=0, trcflg=[]), dict(kdm=41, ntracr=4, trcflg=[2, 0, 0, 0]),
+    dict(hybmap=1, hybiso=3.0),
+])
+def test_device_path_variants(gpu, variant):
+    P, w = _case("cfg1-variant", idm=97, jdm=61, **variant)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=7))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    got = dd.download()
+    _assert_same(want, got, str(variant))
+
+
+@pytest.mark.parametrize("mn", [(1, 2), (2, 1)])
+def test_robert_asselin_and_diag_tracers(gpu, mn):
+    m, n = mn
+    P, w = _case("raflag", idm=80, jdm=50, ntracr=4, trcflg=[2, 0, 0, 0], hybmap=2)
+    P.itracr801, P.itracr802, P.itracr803 = 2, 3, 4
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=11), m=m, n=n)
+    rng = np.random.default_rng(1)
+    d0.a["dpo"][...] = np.abs(d0.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d0.a["dp"].shape)))
+    want = _oracle_out(d0, m=m, n=n, raflag=1)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=m, n=n, raflag=1)
+    got = dd.download()
+    _assert_same(want, got, f"raflag m={m} n={n}")
+
+
+def test_copy_mode_matches_oracle(gpu):
+    P, w = _case("cfg1-copy-mode", hybmap=2)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=3))
+    want = _oracle_out(d0)
+    got = d0.copy().hybgen(m=1, n=2)
+    _assert_same(want, got, "copy mode")
+
+
+def test_device_synth_matches_host_synth(gpu):
+    P, w = _case("synth", idm=130, jdm=77, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0])
+    S = abi.make_synth(P, w, seed=21)
+    h = abi.HostDomain(P).synth(S)
+    dd = abi.DeviceDomain(P).synth(S)
+    got = dd.download()
+    names = [n for n in abi.FIELD_NAMES]
+    df = ol.diff_domains(h, got, names)
+    assert not df, f"generator differs host vs device: {df}"
+
+
+def test_larger_domain_matches_oracle(gpu):
+    P, w = _case("mid-size", idm=600, jdm=400, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0],
+                 workload="glbb")
+    S = abi.make_synth(P, w, seed=200, itdm=4500, jtdm=3298, i0=1800, j0=1200)
+    dd = abi.DeviceDomain(P).synth(S)
+    d0 = dd.download()
+    want = _oracle_out(d0)
+    dd.hybgen(m=1, n=2)
+    got = dd.download()
+    _assert_same(want, got, "600x400x41")
