@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""hybgen throughput on one MI355X.
+
+One step = one  hybgen(m,n,.false.)  call (column regrid+remap of T, S and
+the tracers, p halo, dpudpv + u/v remap) over the whole GLBb-shaped domain,
+fields resident in HBM.  Each call regrids the state the previous call
+left, as successive model steps do.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the column
+kernel, hybgenaj): algorithmic bytes per launch / its mean launch duration
+from HIP events recorded on its own stream.  `cpu_baseline` is the reference
+hybgen (flang build of HYCOM-src, oracle/_ref) timed on this host on a
+bounded window of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hycom-src_amd"))
+import hybgen_abi as abi  # noqa: E402
+
+METRIC = "column-layers remapped/s, FP64 vs mixed, GLBb tile; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+PHASES = ["column", "halo", "uv", "ra", "halo2", "uv2"]
+
+
+def bytes_per_column_phase_a(K, T):
+    """Algorithmic HBM bytes of the column kernel per sea column: read and
+    write dp, T, S, th3d and T tracers (K levels each), read theta (K),
+    write p (K+1), read depths and topiso, read ip (int32)."""
+    return 8 * (2 * (4 + T) * K + K + (K + 1)) + 8 * 2 + 4
+
+
+def bytes_per_column_call(K, T):
+    """Algorithmic bytes of the whole call per sea column (raflag off):
+    8*[(20+2T)K + 10] + 12 (scalar fields, p, u/v points, masks)."""
+    return 8 * ((20 + 2 * T) * K + 10) + 12
+
+
+def sea_columns(dd):
+    import torch
+    P = dd.P
+    NI, NJ = P.idm + 2 * P.nbdy, P.jdm + 2 * P.nbdy
+    ip = dd.t["ip"].view(NJ, NI)[P.nbdy:P.nbdy + P.jj, P.nbdy:P.nbdy + P.ii]
+    return int(torch.count_nonzero(ip).item())
+
+
+def cpu_baseline(workload, hybmap, rows, seed):
+    """Reference hybgen (flang build of the HYCOM-src sources) on a window of
+    `rows` full-width j-rows of the same global field, on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    if not ol.ref_available():
+        return None
+    w0 = abi.WORKLOADS[workload]
+    itdm, jtdm = w0["idm"], w0["jdm"]
+    j0 = max(0, jtdm // 2 - rows // 2)
+    P, w = abi.workload_params(workload, hybmap=hybmap, jdm=rows)
+    S = abi.make_synth(P, w, seed=seed, itdm=itdm, jtdm=jtdm, i0=0, j0=j0)
+    d = abi.HostDomain(P).synth(S)
+    sea = int((d.a["ip"][P.nbdy:P.nbdy + P.ii, P.nbdy:P.nbdy + P.jj] > 0).sum())
+    times = []
+    for _ in range(3):
+        times.append(ol.run_ref(d.copy()))
+    times.sort()
+    t = times[len(times) // 2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return {
+        "value": sea * P.kdm / t,
+        "unit": "column-layers/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"HYCOM-src hybgen (flang -O2, OpenMP over j) on rows {j0 + 1}-{j0 + rows} "
+                   f"of the {itdm}x{jtdm}x{P.kdm} field ({sea} sea columns), median of 3 "
+                   f"calls of {t:.3f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="glbb", choices=sorted(abi.WORKLOADS))
+    ap.add_argument("--hybmap", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=200)
+    ap.add_argument("--cpu-rows", type=int, default=160)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the hybgen path is single-GPU here; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    lib = abi.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    P, w = abi.workload_params(args.workload, hybmap=args.hybmap)
+    print(f"[bench] allocating {args.workload} {P.idm}x{P.jdm}x{P.kdm} ntracr={P.ntracr}",
+          file=sys.stderr, flush=True)
+    dd = abi.DeviceDomain(P, skip=("dpo",))
+    S = abi.make_synth(P, w, seed=args.seed)
+    stream = torch.cuda.current_stream().cuda_stream
+    dd.synth(S, stream=stream)
+    torch.cuda.synchronize()
+    sea = sea_columns(dd)
+    print(f"[bench] sea columns {sea}; warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+
+    np_ = len(PHASES)
+    ms = (ctypes.c_double * np_)()
+    cnt = (ctypes.c_int32 * np_)()
+    abi.check(lib.hycom_hybgen_timing(1))
+    step_s = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        dd.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    abi.check(lib.hycom_hybgen_timing_read(ms, cnt, np_))
+    abi.check(lib.hycom_hybgen_timing(0))
+    print(f"[bench] timed {args.steps} steps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
+
+    total = t1 - t0
+    ms_step = 1e3 * total / args.steps
+    value = sea * P.kdm * args.steps / total
+    ms_a = ms[0] / max(cnt[0], 1)
+    bpc_a = bytes_per_column_phase_a(P.kdm, P.ntracr)
+    achieved = sea * bpc_a / (ms_a * 1e-3) / 1e9
+    srt = sorted(step_s)
+    p50 = srt[len(srt) // 2] * 1e3
+    p99 = srt[min(len(srt) - 1, int(round(0.99 * (len(srt) - 1))))] * 1e3
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_hybmap{args.hybmap}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.workload, args.hybmap, args.cpu_rows, args.seed)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "column-layers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, GLBb-shaped bathymetry and stratification)",
+        "config": {
+            "workload": f"{args.workload}-{P.idm}x{P.jdm}x{P.kdm}-"
+                        f"{['pcm', 'plm', 'ppm', 'weno'][args.hybmap]}",
+            "idm": P.idm, "jdm": P.jdm, "kdm": P.kdm, "ntracr": P.ntracr,
+            "hybmap": args.hybmap, "precision": "fp64", "sea_columns": sea,
+            "hybgen_raflag": False,
+        },
+        "latency_ms": {"p50": p50, "p99": p99},
+        "phase_ms": {PHASES[q]: ms[q] / cnt[q] for q in range(np_) if cnt[q]},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_phase_a (column regrid + remap, hybgenaj)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": sea * bpc_a,
+            "bytes_per_sea_column": bpc_a,
+        },
+        "call_bytes_per_sea_column": bytes_per_column_call(P.kdm, P.ntracr),
+        "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -114,6 +114,31 @@ struct Mirror {
 std::mutex g_mirror_mu;
 Mirror g_mirror;
 
+// measurement: per-phase HIP events of the calls since the last read
+std::mutex g_timing_mu;
+bool g_timing_on = false;
+std::vector<PhaseTimer*> g_timers_pending, g_timers_free;
+
+PhaseTimer* timer_take() {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  if (!g_timing_on) return nullptr;
+  PhaseTimer* t;
+  if (!g_timers_free.empty()) {
+    t = g_timers_free.back();
+    g_timers_free.pop_back();
+  } else {
+    t = new PhaseTimer;
+    for (int q = 0; q < kPhases; q++) {
+      if (hipEventCreate(&t->start[q]) != hipSuccess || hipEventCreate(&t->stop[q]) != hipSuccess) {
+        delete t;
+        return nullptr;
+      }
+    }
+  }
+  g_timers_pending.push_back(t);
+  return t;
+}
+
 thread_local int32_t* g_rec_fixlay = nullptr;
 thread_local int32_t* g_rec_kp = nullptr;
 
@@ -275,7 +300,31 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   View V = view_of(P, F);
   V.rec_fixlay = g_rec_fixlay;
   V.rec_kp = g_rec_kp;
-  HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream));
+  HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream, timer_take()));
+  return 0;
+}
+
+int hycom_hybgen_timing(int enable) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing_on = enable != 0;
+  return 0;
+}
+
+int hycom_hybgen_timing_read(double* ms_sum, int32_t* count, int nphase) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  for (int q = 0; q < nphase; q++) { ms_sum[q] = 0.0; count[q] = 0; }
+  for (PhaseTimer* t : g_timers_pending) {
+    for (int q = 0; q < kPhases && q < nphase; q++) {
+      if (!t->used[q]) continue;
+      HIPCHK(hipEventSynchronize(t->stop[q]));
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, t->start[q], t->stop[q]));
+      ms_sum[q] += ms;
+      count[q] += 1;
+    }
+    g_timers_free.push_back(t);
+  }
+  g_timers_pending.clear();
   return 0;
 }
 

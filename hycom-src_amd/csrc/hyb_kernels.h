@@ -8,9 +8,18 @@
 
 namespace hyb {
 
+// Optional HIP events around each phase of one call, for measurement:
+// 0 column kernel (hybgenaj), 1 p halo, 2 u/v kernel (dpudpv+hybgenbj),
+// 3 Robert-Asselin kernel (hybgencj), 4 second halo, 5 second u/v kernel.
+constexpr int kPhases = 6;
+struct PhaseTimer {
+  hipEvent_t start[kPhases], stop[kPhases];
+  bool used[kPhases];
+};
+
 // Enqueues the whole FP64 hybgen(m,n,raflag) on stream s.
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,
-                              int raflag, hipStream_t s);
+                              int raflag, hipStream_t s, PhaseTimer* T = nullptr);
 
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,

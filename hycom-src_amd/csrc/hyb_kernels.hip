@@ -94,9 +94,23 @@ static hipError_t launch_halo(const KParams& P, const View& V, hipStream_t s) {
   return hipGetLastError();
 }
 
+struct PhaseMark {
+  PhaseTimer* T;
+  hipStream_t s;
+  int ph;
+  PhaseMark(PhaseTimer* t, hipStream_t st, int p) : T(t), s(st), ph(p) {
+    if (T) { (void)hipEventRecord(T->start[ph], s); T->used[ph] = true; }
+  }
+  ~PhaseMark() {
+    if (T) (void)hipEventRecord(T->stop[ph], s);
+  }
+};
+
 template <int KC>
 static hipError_t launch_all(const KParams& P, const View& V, int m, int n,
-                             int raflag, hipStream_t s) {
+                             int raflag, hipStream_t s, PhaseTimer* T) {
+  if (T)
+    for (int q = 0; q < kPhases; q++) T->used[q] = false;
   dim3 blk(kBlock);
   dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
   dim3 g3((P.ii + kBlock - 1) / kBlock, P.jj, 2);
@@ -104,29 +118,46 @@ static hipError_t launch_all(const KParams& P, const View& V, int m, int n,
   const double* src[3] = {V.dp, V.temp, V.saln};
   for (int d = 0; d < 3; d++)
     if (itr[d] > 0) hipLaunchKernelGGL(k_diag_pre, g2, blk, 0, s, P, V, n, itr[d], src[d]);
-  hipLaunchKernelGGL(k_phase_a<KC>, g2, blk, 0, s, P, V, n);
+  {
+    PhaseMark pm(T, s, 0);
+    hipLaunchKernelGGL(k_phase_a<KC>, g2, blk, 0, s, P, V, n);
+  }
   for (int d = 0; d < 3; d++)
     if (itr[d] > 0)
       hipLaunchKernelGGL(k_diag_post, g2, blk, 0, s, P, V, n, itr[d], src[d],
                          d == 0 ? P.diag_q801 : P.diag_q80x);
-  hipError_t e = launch_halo(P, V, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_phase_c<KC>, g3, blk, 0, s, P, V, n);
-  if (raflag) {
-    hipLaunchKernelGGL(k_phase_ra<KC>, g2, blk, 0, s, P, V, m, n);
+  hipError_t e;
+  {
+    PhaseMark pm(T, s, 1);
     e = launch_halo(P, V, s);
+  }
+  if (e != hipSuccess) return e;
+  {
+    PhaseMark pm(T, s, 2);
+    hipLaunchKernelGGL(k_phase_c<KC>, g3, blk, 0, s, P, V, n);
+  }
+  if (raflag) {
+    {
+      PhaseMark pm(T, s, 3);
+      hipLaunchKernelGGL(k_phase_ra<KC>, g2, blk, 0, s, P, V, m, n);
+    }
+    {
+      PhaseMark pm(T, s, 4);
+      e = launch_halo(P, V, s);
+    }
     if (e != hipSuccess) return e;
+    PhaseMark pm(T, s, 5);
     hipLaunchKernelGGL(k_phase_c<KC>, g3, blk, 0, s, P, V, m);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,
-                              int raflag, hipStream_t s) {
-  if (P.kk <= 24) return launch_all<24>(P, V, m, n, raflag, s);
-  if (P.kk <= 32) return launch_all<32>(P, V, m, n, raflag, s);
-  if (P.kk <= 48) return launch_all<48>(P, V, m, n, raflag, s);
-  return launch_all<64>(P, V, m, n, raflag, s);
+                              int raflag, hipStream_t s, PhaseTimer* T) {
+  if (P.kk <= 24) return launch_all<24>(P, V, m, n, raflag, s, T);
+  if (P.kk <= 32) return launch_all<32>(P, V, m, n, raflag, s, T);
+  if (P.kk <= 48) return launch_all<48>(P, V, m, n, raflag, s, T);
+  return launch_all<64>(P, V, m, n, raflag, s, T);
 }
 
 // ---------------------------------------------------------------------

@@ -74,6 +74,9 @@ SIGNATURES = {
     "hycom_hybgen_device": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "hycom_hybgen_timing": (ctypes.c_int, [ctypes.c_int]),
+    "hycom_hybgen_timing_read": (ctypes.c_int, [_P(ctypes.c_double), _P(ctypes.c_int32),
+                                                ctypes.c_int]),
     "hycom_hybgen_synth_host": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_synth_device": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
@@ -257,18 +260,20 @@ class HostDomain:
 class DeviceDomain:
     """All hybgen fields of one domain as flat device tensors."""
 
-    def __init__(self, P, device="cuda"):
+    def __init__(self, P, device="cuda", skip=()):
         import torch
         self.P = P
         self.t = {}
         for n, s in field_shapes(P).items():
+            if n in skip:
+                continue
             dt = torch.int32 if n in INT_ARRAYS else torch.float64
             self.t[n] = torch.zeros(int(np.prod(s)), dtype=dt, device=device)
 
     def fields(self):
         F = HybFields()
         for n in FIELD_NAMES:
-            setattr(F, n, self.t[n].data_ptr())
+            setattr(F, n, self.t[n].data_ptr() if n in self.t else None)
         if self.P.ntracr == 0:
             F.tracer = None
         return F
@@ -276,13 +281,15 @@ class DeviceDomain:
     def upload(self, host):
         import torch
         for n, v in host.a.items():
+            if n not in self.t:
+                continue
             self.t[n].copy_(torch.from_numpy(np.ascontiguousarray(v.reshape(-1, order="F"))))
         return self
 
     def download(self, names=None):
         shapes = field_shapes(self.P)
         out = {}
-        for n in (names or shapes.keys()):
+        for n in (names or [k for k in shapes if k in self.t]):
             out[n] = self.t[n].cpu().numpy().reshape(shapes[n], order="F")
         return HostDomain(self.P, out) if names is None else out
 

@@ -171,6 +171,16 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
  * pointers; NULL disables recording (the default). */
 int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev);
 
+/* Measurement hooks (no reference counterpart; the reference's only timer
+ * is xctmr 49 around the whole call, mod_hycom.F90:2648-2653).  While
+ * enabled, every hycom_hybgen_device() call records HIP events around its
+ * phases on its stream: 0 column kernel (hybgenaj), 1 p halo, 2 u/v kernel
+ * (dpudpv + hybgenbj), 3 Robert-Asselin kernel (hybgencj), 4 second halo,
+ * 5 second u/v kernel.  _read waits for the recorded calls, returns the
+ * summed milliseconds and call counts per phase, and forgets them. */
+int hycom_hybgen_timing(int enable);
+int hycom_hybgen_timing_read(double* ms_sum, int32_t* count, int nphase);
+
 /* ---------------------------------------------------------------------- */
 /* Synthetic inputs (workload generator; no reference counterpart)         */
 /* ---------------------------------------------------------------------- */

@@ -517,7 +517,7 @@ void oracle_remap_column(int method, int kk, const double* si,
                          double thin, double* so) {
   static __thread colv s[2], f[2], c[8];
   double pres[KD], prsf[KD], dprs[KD], dpi[KD];
-  int lcm[KD];
+  int lcm[KD] = {0};
   for (int k = 1; k <= kk; k++) {
     s[1][k] = si[k - 1];
     lcm[k] = lcm0 ? lcm0[k - 1] : 0;
@@ -570,7 +570,7 @@ static void hybgenaj(const ctx_t* c, int m, int n, int j) {
    fmax2(1.0, (delp) / ((dp0) * qqmx)))
 
   static __thread colv s1d[HYB_TRMAX + 5], f1d[HYB_TRMAX + 5], c1d[3 * (HYB_TRMAX + 5)];
-  int lcm[KD];
+  int lcm[KD] = {0};
   double dpi[KD], dprs[KD], pres[KD], prsf[KD], qhrlx[KD], dp0ij[KD], dp0cum[KD];
 
   const double dpthin = 0.001 * ONEMM;
@@ -1090,7 +1090,7 @@ static void hybgencj(const ctx_t* c, int m, int n, int j) {
   double *p = F->p, *dp = F->dp, *dpo = F->dpo, *temp = F->temp, *saln = F->saln, *th3d = F->th3d;
   const double* theta = F->theta;
   static __thread colv s1d[HYB_TRMAX + 5], f1d[HYB_TRMAX + 5], c1d[3 * (HYB_TRMAX + 5)];
-  int lcm[KD];
+  int lcm[KD] = {0};
   double dpi[KD], dprs[KD], pres[KD], prsf[KD], dp0ij[KD], dp0cum[KD];
   const double dpthin = 0.001 * ONEMM;
   const int nums1d = ntracr + 2;
@@ -1165,7 +1165,7 @@ static void hybgenb_point(const ctx_t* c, int nl, int i, int j, double* pu,
   const hyb_params* P = c->P;
   const int kk = c->K;
   colv s1d[2], f1d[2], c1d[8];
-  int lcm[KD];
+  int lcm[KD] = {0};
   double dpi[KD], dprs[KD], pres[KD], prsf[KD];
   const double dpthin = 0.001 * ONEMM;
   for (int k = 1; k <= kk; k++) lcm[k] = 0;
