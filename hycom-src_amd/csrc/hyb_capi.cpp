@@ -139,6 +139,49 @@ PhaseTimer* timer_take() {
   return t;
 }
 
+// Workspace of the register-resident kernels (pres, kp, lcm), one per
+// domain, keyed by the domain's p array; allocated on first use.
+struct Workspace {
+  const void* key = nullptr;
+  int64_t nij = 0;
+  int K = 0;
+  double* pres = nullptr;
+  int32_t* kp = nullptr;
+  uint64_t* lcm = nullptr;
+};
+std::mutex g_ws_mu;
+std::vector<Workspace> g_ws;
+int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays only, 2 register-resident only
+
+int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto& w : g_ws) {
+    if (w.key == key && w.nij == nij && w.K == K) {
+      *W = Scratch{w.pres, w.kp, w.lcm};
+      return 0;
+    }
+  }
+  for (auto it = g_ws.begin(); it != g_ws.end(); ++it) {
+    if (it->key == key) {  // same domain, new shape
+      (void)hipFree(it->pres);
+      (void)hipFree(it->kp);
+      (void)hipFree(it->lcm);
+      g_ws.erase(it);
+      break;
+    }
+  }
+  Workspace w;
+  w.key = key;
+  w.nij = nij;
+  w.K = K;
+  HIPCHK(hipMalloc(&w.pres, sizeof(double) * nij * (K + 1)));
+  HIPCHK(hipMalloc(&w.kp, sizeof(int32_t) * nij));
+  HIPCHK(hipMalloc(&w.lcm, sizeof(uint64_t) * nij));
+  g_ws.push_back(w);
+  *W = Scratch{w.pres, w.kp, w.lcm};
+  return 0;
+}
+
 thread_local int32_t* g_rec_fixlay = nullptr;
 thread_local int32_t* g_rec_kp = nullptr;
 
@@ -300,7 +343,23 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   View V = view_of(P, F);
   V.rec_fixlay = g_rec_fixlay;
   V.rec_kp = g_rec_kp;
-  HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream, timer_take()));
+  bool fast = fast_path_ok(K) && g_kernel_path != 1;
+  if (g_kernel_path == 2 && !fast)
+    return fail(HYB_EUNSUPPORTED, "register-resident kernels do not cover kdm=%d/ldenfix/nhybrd",
+                P->kdm);
+  if (fast) {
+    Scratch W;
+    if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
+    HIPCHK(launch_hybgen_fast(K, V, W, m, n, raflag, (hipStream_t)stream, timer_take()));
+  } else {
+    HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream, timer_take()));
+  }
+  return 0;
+}
+
+int hycom_hybgen_set_kernel_path(int path) {
+  if (path < 0 || path > 2) return fail(HYB_EINVAL, "kernel path %d", path);
+  g_kernel_path = path;
   return 0;
 }
 

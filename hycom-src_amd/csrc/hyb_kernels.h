@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include "hyb_column.h"
+#include "hyb_fast.h"
 #include "hyb_synth.h"
 
 namespace hyb {
@@ -20,6 +21,13 @@ struct PhaseTimer {
 // Enqueues the whole FP64 hybgen(m,n,raflag) on stream s.
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,
                               int raflag, hipStream_t s, PhaseTimer* T = nullptr);
+
+// True when the register-resident kernels cover this parameter pack.
+bool fast_path_ok(const KParams& P);
+// Enqueues hybgen with the register-resident kernels; W must hold
+// NIJ*(kk+1) doubles (pres), NIJ int32 (kp) and NIJ uint64 (lcm).
+hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W, int m,
+                              int n, int raflag, hipStream_t s, PhaseTimer* T = nullptr);
 
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hyb_column.h"
+#include "hyb_fast.h"
 #include "hyb_kernels.h"
 #include "hyb_synth.h"
 
@@ -47,6 +48,44 @@ __global__ void __launch_bounds__(kBlock) k_phase_c(KParams P, View V, int nl) {
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[ij_of(V, i, j)] == 0) return;
   point_c<KC>(P, V, i, j, nl, isv);
+}
+
+// ---------------------------------------------------------------------
+// register-resident kernels for fixed K (hyb_fast.h)
+// ---------------------------------------------------------------------
+constexpr int kWave = 64;
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_regrid(KParams P, View V, Scratch W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  if (V.ip[ij_of(V, i, j)] == 0) return;
+  regrid_fast<K>(P, V, W, i, j, n);
+}
+
+template <int K, int METHOD>
+__global__ void __launch_bounds__(kWave) k_remap(KParams P, View V, Scratch W, int n) {
+  extern __shared__ double lds[];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  if (V.ip[ij_of(V, i, j)] == 0) return;
+  LaneRow row{lds + threadIdx.x, kWave};
+  remap_fast<K, METHOD>(P, V, W, i, j, n, row);
+}
+
+template <int K>
+__global__ void __launch_bounds__(kWave) k_uv(KParams P, View V, int nl) {
+  extern __shared__ double lds[];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  bool isv = blockIdx.z != 0;
+  if (i > P.ii) return;
+  const int32_t* msk = isv ? V.iv : V.iu;
+  if (msk[ij_of(V, i, j)] == 0) return;
+  LaneRow row{lds + threadIdx.x, kWave};
+  uv_fast<K>(P, V, i, j, nl, isv, row);
 }
 
 // p halo, levels 2..kk+1: first the j rows (mod_xc_sm.h:1374-1392), then
@@ -150,6 +189,76 @@ static hipError_t launch_all(const KParams& P, const View& V, int m, int n,
     hipLaunchKernelGGL(k_phase_c<KC>, g3, blk, 0, s, P, V, m);
   }
   return hipGetLastError();
+}
+
+template <int K>
+static hipError_t launch_fast(const KParams& P, const View& V, const Scratch& W, int m,
+                              int n, int raflag, hipStream_t s, PhaseTimer* T) {
+  if (T)
+    for (int q = 0; q < kPhases; q++) T->used[q] = false;
+  dim3 blk(kBlock);
+  dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
+  dim3 w1(kWave);
+  dim3 gw2((P.ii + kWave - 1) / kWave, P.jj);
+  dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
+  const size_t lds = sizeof(double) * (K + 1) * kWave;
+  const int itr[3] = {P.itr801, P.itr802, P.itr803};
+  const double* src[3] = {V.dp, V.temp, V.saln};
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0) hipLaunchKernelGGL(k_diag_pre, g2, blk, 0, s, P, V, n, itr[d], src[d]);
+  {
+    PhaseMark pm(T, s, 0);
+    hipLaunchKernelGGL(k_regrid<K>, g2, blk, 0, s, P, V, W, n);
+    switch (P.hybmap) {
+      case 0: hipLaunchKernelGGL((k_remap<K, 0>), gw2, w1, lds, s, P, V, W, n); break;
+      case 1: hipLaunchKernelGGL((k_remap<K, 1>), gw2, w1, lds, s, P, V, W, n); break;
+      case 2: hipLaunchKernelGGL((k_remap<K, 2>), gw2, w1, lds, s, P, V, W, n); break;
+      default: hipLaunchKernelGGL((k_remap<K, 3>), gw2, w1, lds, s, P, V, W, n); break;
+    }
+  }
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0)
+      hipLaunchKernelGGL(k_diag_post, g2, blk, 0, s, P, V, n, itr[d], src[d],
+                         d == 0 ? P.diag_q801 : P.diag_q80x);
+  hipError_t e;
+  {
+    PhaseMark pm(T, s, 1);
+    e = launch_halo(P, V, s);
+  }
+  if (e != hipSuccess) return e;
+  {
+    PhaseMark pm(T, s, 2);
+    hipLaunchKernelGGL(k_uv<K>, gw3, w1, lds, s, P, V, n);
+  }
+  if (raflag) {
+    {
+      PhaseMark pm(T, s, 3);
+      hipLaunchKernelGGL(k_phase_ra<K>, g2, blk, 0, s, P, V, m, n);
+    }
+    {
+      PhaseMark pm(T, s, 4);
+      e = launch_halo(P, V, s);
+    }
+    if (e != hipSuccess) return e;
+    PhaseMark pm(T, s, 5);
+    hipLaunchKernelGGL(k_uv<K>, gw3, w1, lds, s, P, V, m);
+  }
+  return hipGetLastError();
+}
+
+bool fast_path_ok(const KParams& P) {
+  if (P.ldenfix || P.nhybrd != P.kk) return false;
+  return P.kk == 22 || P.kk == 32 || P.kk == 41;
+}
+
+hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W, int m,
+                              int n, int raflag, hipStream_t s, PhaseTimer* T) {
+  switch (P.kk) {
+    case 22: return launch_fast<22>(P, V, W, m, n, raflag, s, T);
+    case 32: return launch_fast<32>(P, V, W, m, n, raflag, s, T);
+    case 41: return launch_fast<41>(P, V, W, m, n, raflag, s, T);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,

@@ -73,3 +73,87 @@ extern "C" int hostcheck_hybgen(const hyb_params* P, const hyb_fields* F, int m,
   else run<64>(K, V, m, n, raflag);
   return 0;
 }
+
+// ---- register-resident kernels (hyb_fast.h), same phase order ----
+#include "hyb_fast.h"
+#include <vector>
+
+template <int K>
+static void run_fast(const KParams& P, const View& V, int m, int n, int raflag) {
+  const int ii = P.ii, jj = P.jj, kk = P.kk;
+  std::vector<double> pres((size_t)V.NIJ * (K + 1));
+  std::vector<int32_t> kpv((size_t)V.NIJ);
+  std::vector<uint64_t> lcmv((size_t)V.NIJ);
+  Scratch W{pres.data(), kpv.data(), lcmv.data()};
+  const int itr[3] = {P.itr801, P.itr802, P.itr803};
+  double* src[3] = {V.dp, V.temp, V.saln};
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          for (int k = 1; k <= kk; k++)
+            V.tracer[ij_of(V, i, j) + levt(V, k, n, itr[d])] = src[d][ij_of(V, i, j) + lev4(V, k, n)];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 1; j <= jj; j++)
+    for (int i = 1; i <= ii; i++)
+      if (V.ip[ij_of(V, i, j)]) regrid_fast<K>(P, V, W, i, j, n);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 1; j <= jj; j++) {
+    double row[K + 2];
+    LaneRow lr{row, 1};
+    for (int i = 1; i <= ii; i++) {
+      if (!V.ip[ij_of(V, i, j)]) continue;
+      switch (P.hybmap) {
+        case 0: remap_fast<K, 0>(P, V, W, i, j, n, lr); break;
+        case 1: remap_fast<K, 1>(P, V, W, i, j, n, lr); break;
+        case 2: remap_fast<K, 2>(P, V, W, i, j, n, lr); break;
+        default: remap_fast<K, 3>(P, V, W, i, j, n, lr); break;
+      }
+    }
+  }
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0) {
+      double q = d == 0 ? P.diag_q801 : P.diag_q80x;
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)])
+            for (int k = 1; k <= kk; k++) {
+              int64_t a = ij_of(V, i, j) + levt(V, k, n, itr[d]);
+              V.tracer[a] = q * (src[d][ij_of(V, i, j) + lev4(V, k, n)] - V.tracer[a]);
+            }
+    }
+  for (int pass = 0; pass <= (raflag ? 1 : 0); pass++) {
+    int nl = pass == 0 ? n : m;
+    if (pass == 1) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)]) column_ra<K>(P, V, i, j, m, n, row_lcm(P));
+    }
+    for (int k = 2; k <= kk + 1; k++)
+      for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
+    for (int k = 2; k <= kk + 1; k++)
+      for (int j = 0; j <= jj + 1; j++) p_halo_icol(P, V, j, k);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 1; j <= jj; j++) {
+      double row[K + 2];
+      LaneRow lr{row, 1};
+      for (int isv = 0; isv <= 1; isv++)
+        for (int i = 1; i <= ii; i++)
+          if ((isv ? V.iv : V.iu)[ij_of(V, i, j)]) uv_fast<K>(P, V, i, j, nl, isv != 0, lr);
+    }
+  }
+}
+
+extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, int m, int n,
+                                     int raflag) {
+  KParams K = kparams_of(P);
+  View V = view_of(P, F);
+  if (K.ldenfix || K.nhybrd != K.kk) return -2;
+  switch (K.kk) {
+    case 22: run_fast<22>(K, V, m, n, raflag); return 0;
+    case 32: run_fast<32>(K, V, m, n, raflag); return 0;
+    case 41: run_fast<41>(K, V, m, n, raflag); return 0;
+    default: return -2;
+  }
+}

@@ -12,6 +12,16 @@ import oracle_lib as ol
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=[1, 2], ids=["lane-arrays", "register-resident"])
+def kernel_path(request, gpu):
+    """Runs the test once per kernel family (the register-resident kernels
+    cover kdm in {22,32,41}; other shapes fall back inside the library)."""
+    lib = abi.load()
+    abi.check(lib.hycom_hybgen_set_kernel_path(0 if request.param == 2 else 1))
+    yield request.param
+    abi.check(lib.hycom_hybgen_set_kernel_path(0))
+
+
 def _case(name, **over):
     print(f"\n[case] {name} {over}", flush=True)
     P, w = abi.workload_params(over.pop("workload", "cfg1"), **over)
@@ -30,7 +40,7 @@ def _assert_same(a, b, what):
 
 
 @pytest.mark.parametrize("hybmap", [0, 1, 2, 3])
-def test_device_path_matches_oracle_cfg1(gpu, hybmap):
+def test_device_path_matches_oracle_cfg1(gpu, kernel_path, hybmap):
     P, w = _case("cfg1-device", hybmap=hybmap)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=100))
     want = _oracle_out(d0)
@@ -46,9 +56,10 @@ def test_device_path_matches_oracle_cfg1(gpu, hybmap):
     dict(ldenfix=1), dict(hybthk=0.5, hybthn=5.0), dict(nsigma=0, hybrlx=1.0),
     dict(nreg=1, dx00=30.0, dx00x=300.0, dx00f=1.1), dict(nreg=3, hybmap=3),
     dict(kdm=22, ntracr=0, trcflg=[]), dict(kdm=41, ntracr=4, trcflg=[2, 0, 0, 0]),
-    dict(hybmap=1, hybiso=3.0),
+    dict(hybmap=1, hybiso=3.0), dict(kdm=41, hybmap=3, ntracr=1, trcflg=[0]),
+    dict(kdm=22, hybmap=0, nreg=1), dict(kdm=41, hybmap=1, hybthk=0.9),
 ])
-def test_device_path_variants(gpu, variant):
+def test_device_path_variants(gpu, kernel_path, variant):
     P, w = _case("cfg1-variant", idm=97, jdm=61, **variant)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=7))
     want = _oracle_out(d0)
@@ -59,7 +70,7 @@ def test_device_path_variants(gpu, variant):
 
 
 @pytest.mark.parametrize("mn", [(1, 2), (2, 1)])
-def test_robert_asselin_and_diag_tracers(gpu, mn):
+def test_robert_asselin_and_diag_tracers(gpu, kernel_path, mn):
     m, n = mn
     P, w = _case("raflag", idm=80, jdm=50, ntracr=4, trcflg=[2, 0, 0, 0], hybmap=2)
     P.itracr801, P.itracr802, P.itracr803 = 2, 3, 4
@@ -92,7 +103,7 @@ def test_device_synth_matches_host_synth(gpu):
     assert not df, f"generator differs host vs device: {df}"
 
 
-def test_larger_domain_matches_oracle(gpu):
+def test_larger_domain_matches_oracle(gpu, kernel_path):
     P, w = _case("mid-size", idm=600, jdm=400, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0],
                  workload="glbb")
     S = abi.make_synth(P, w, seed=200, itdm=4500, jtdm=3298, i0=1800, j0=1200)
