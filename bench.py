@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--seed", type=int, default=200)
     ap.add_argument("--cpu-rows", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-path", type=int, default=0,
+                    help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,6 +102,7 @@ def main():
 
     import torch
     lib = abi.load()
+    abi.check(lib.hycom_hybgen_set_kernel_path(args.kernel_path))
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     P, w = abi.workload_params(args.workload, hybmap=args.hybmap)

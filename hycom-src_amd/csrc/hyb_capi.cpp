@@ -151,7 +151,7 @@ struct Workspace {
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
-int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays only, 2 register-resident only
+int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays, 2 register-resident, 3 streaming
 
 int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
@@ -343,11 +343,19 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   View V = view_of(P, F);
   V.rec_fixlay = g_rec_fixlay;
   V.rec_kp = g_rec_kp;
-  bool fast = fast_path_ok(K) && g_kernel_path != 1;
-  if (g_kernel_path == 2 && !fast)
+  const int path = g_kernel_path;
+  if (path == 2 && !fast_path_ok(K))
     return fail(HYB_EUNSUPPORTED, "register-resident kernels do not cover kdm=%d/ldenfix/nhybrd",
                 P->kdm);
-  if (fast) {
+  if (path == 3 && !stream_path_ok(K))
+    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
+  const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
+  const bool fast = path == 2;
+  if (use_stream) {
+    Scratch W;
+    if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
+    HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, (hipStream_t)stream, timer_take()));
+  } else if (fast) {
     Scratch W;
     if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
     HIPCHK(launch_hybgen_fast(K, V, W, m, n, raflag, (hipStream_t)stream, timer_take()));
@@ -358,7 +366,7 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
 }
 
 int hycom_hybgen_set_kernel_path(int path) {
-  if (path < 0 || path > 2) return fail(HYB_EINVAL, "kernel path %d", path);
+  if (path < 0 || path > 3) return fail(HYB_EINVAL, "kernel path %d", path);
   g_kernel_path = path;
   return 0;
 }

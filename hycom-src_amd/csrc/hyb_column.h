@@ -21,6 +21,7 @@ struct KParams {
   double thbase, hybrlx, hybiso, hybthn, hybthk, thkbop, dp00i, dpns, dsns;
   double ra2fac, vland, diag_q801, diag_q80x;
   double dp0k[64], ds0k[64], dx0k[64];
+  double dp0cum[65];  // dp0cum(k), k=1..kk+1, of columns with qdep==1
   uint64_t trc_t;  // bit t-1 set when trcflg(t)==2 (tracer follows T)
 };
 

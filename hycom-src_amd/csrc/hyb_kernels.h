@@ -29,6 +29,13 @@ bool fast_path_ok(const KParams& P);
 hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W, int m,
                               int n, int raflag, hipStream_t s, PhaseTimer* T = nullptr);
 
+// True when the streaming kernels cover this parameter pack (any kdm,
+// nhybrd==kdm, no HYBGEN_DENFIX).
+bool stream_path_ok(const KParams& P);
+// Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast.
+hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
+                                int n, int raflag, hipStream_t s, PhaseTimer* T = nullptr);
+
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,
                         int m, int n, hipStream_t s);

@@ -6,6 +6,7 @@
 #include "hyb_column.h"
 #include "hyb_fast.h"
 #include "hyb_kernels.h"
+#include "hyb_stream.h"
 #include "hyb_synth.h"
 
 namespace hyb {
@@ -86,6 +87,45 @@ __global__ void __launch_bounds__(kWave) k_uv(KParams P, View V, int nl) {
   if (msk[ij_of(V, i, j)] == 0) return;
   LaneRow row{lds + threadIdx.x, kWave};
   uv_fast<K>(P, V, i, j, nl, isv, row);
+}
+
+// ---------------------------------------------------------------------
+// streaming kernels (hyb_stream.h), any kdm
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  if (V.ip[ij_of(V, i, j)] == 0) return;
+  regrid_stream(P, V, W, i, j, n);
+}
+
+// one wave per block; each lane keeps its column's new interfaces in its
+// own LDS column (stride 64 doubles: conflict-free for any per-lane k)
+template <int METHOD>
+__global__ void __launch_bounds__(kWave) k_remap_s(KParams P, View V, Scratch W, int n) {
+  extern __shared__ double lds[];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  if (V.ip[c0] == 0) return;
+  if (W.kp[c0] & kUnsafe) return;
+  double* row = lds + threadIdx.x;
+  const double* pg = V.p + c0;
+  for (int k = 1; k <= P.kk + 1; k++) row[(k - 1) * kWave] = pg[(int64_t)(k - 1) * V.NIJ];
+  LaneRow po{row, kWave};
+  remap_stream<METHOD>(P, V, W, i, j, n, po);
+}
+
+__global__ void __launch_bounds__(kWave) k_remap_fb(KParams P, View V, Scratch W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  if (V.ip[c0] == 0) return;
+  if (!(W.kp[c0] & kUnsafe)) return;
+  remap_fallback(P, V, W, i, j, n);
 }
 
 // p halo, levels 2..kk+1: first the j rows (mod_xc_sm.h:1374-1392), then
@@ -242,6 +282,89 @@ static hipError_t launch_fast(const KParams& P, const View& V, const Scratch& W,
     if (e != hipSuccess) return e;
     PhaseMark pm(T, s, 5);
     hipLaunchKernelGGL(k_uv<K>, gw3, w1, lds, s, P, V, m);
+  }
+  return hipGetLastError();
+}
+
+static void launch_uv(const KParams& P, const View& V, int nl, hipStream_t s) {
+  dim3 blk(kBlock);
+  dim3 g3((P.ii + kBlock - 1) / kBlock, P.jj, 2);
+  dim3 w1(kWave);
+  dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
+  const size_t lds = sizeof(double) * (P.kk + 1) * kWave;
+  switch (P.kk) {
+    case 22: hipLaunchKernelGGL(k_uv<22>, gw3, w1, lds, s, P, V, nl); return;
+    case 32: hipLaunchKernelGGL(k_uv<32>, gw3, w1, lds, s, P, V, nl); return;
+    case 41: hipLaunchKernelGGL(k_uv<41>, gw3, w1, lds, s, P, V, nl); return;
+    default: break;
+  }
+  if (P.kk <= 24) hipLaunchKernelGGL(k_phase_c<24>, g3, blk, 0, s, P, V, nl);
+  else if (P.kk <= 32) hipLaunchKernelGGL(k_phase_c<32>, g3, blk, 0, s, P, V, nl);
+  else if (P.kk <= 48) hipLaunchKernelGGL(k_phase_c<48>, g3, blk, 0, s, P, V, nl);
+  else hipLaunchKernelGGL(k_phase_c<64>, g3, blk, 0, s, P, V, nl);
+}
+
+static void launch_ra(const KParams& P, const View& V, int m, int n, hipStream_t s) {
+  dim3 blk(kBlock);
+  dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
+  if (P.kk <= 24) hipLaunchKernelGGL(k_phase_ra<24>, g2, blk, 0, s, P, V, m, n);
+  else if (P.kk <= 32) hipLaunchKernelGGL(k_phase_ra<32>, g2, blk, 0, s, P, V, m, n);
+  else if (P.kk <= 48) hipLaunchKernelGGL(k_phase_ra<48>, g2, blk, 0, s, P, V, m, n);
+  else hipLaunchKernelGGL(k_phase_ra<64>, g2, blk, 0, s, P, V, m, n);
+}
+
+bool stream_path_ok(const KParams& P) { return !P.ldenfix && P.nhybrd == P.kk; }
+
+hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
+                                int n, int raflag, hipStream_t s, PhaseTimer* T) {
+  if (T)
+    for (int q = 0; q < kPhases; q++) T->used[q] = false;
+  dim3 blk(kBlock);
+  dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
+  dim3 w1(kWave);
+  dim3 gw2((P.ii + kWave - 1) / kWave, P.jj);
+  const size_t lds = sizeof(double) * (P.kk + 1) * kWave;
+  const int itr[3] = {P.itr801, P.itr802, P.itr803};
+  const double* src[3] = {V.dp, V.temp, V.saln};
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0) hipLaunchKernelGGL(k_diag_pre, g2, blk, 0, s, P, V, n, itr[d], src[d]);
+  {
+    PhaseMark pm(T, s, 0);
+    hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
+    switch (P.hybmap) {
+      case 0: hipLaunchKernelGGL(k_remap_s<0>, gw2, w1, lds, s, P, V, W, n); break;
+      case 1: hipLaunchKernelGGL(k_remap_s<1>, gw2, w1, lds, s, P, V, W, n); break;
+      case 2: hipLaunchKernelGGL(k_remap_s<2>, gw2, w1, lds, s, P, V, W, n); break;
+      default: hipLaunchKernelGGL(k_remap_s<3>, gw2, w1, lds, s, P, V, W, n); break;
+    }
+    hipLaunchKernelGGL(k_remap_fb, gw2, w1, 0, s, P, V, W, n);
+  }
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0)
+      hipLaunchKernelGGL(k_diag_post, g2, blk, 0, s, P, V, n, itr[d], src[d],
+                         d == 0 ? P.diag_q801 : P.diag_q80x);
+  hipError_t e;
+  {
+    PhaseMark pm(T, s, 1);
+    e = launch_halo(P, V, s);
+  }
+  if (e != hipSuccess) return e;
+  {
+    PhaseMark pm(T, s, 2);
+    launch_uv(P, V, n, s);
+  }
+  if (raflag) {
+    {
+      PhaseMark pm(T, s, 3);
+      launch_ra(P, V, m, n, s);
+    }
+    {
+      PhaseMark pm(T, s, 4);
+      e = launch_halo(P, V, s);
+    }
+    if (e != hipSuccess) return e;
+    PhaseMark pm(T, s, 5);
+    launch_uv(P, V, m, s);
   }
   return hipGetLastError();
 }

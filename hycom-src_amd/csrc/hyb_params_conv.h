@@ -44,6 +44,11 @@ inline KParams kparams_of(const hyb_params* P) {
     K.ds0k[k] = P->ds0k[k];
     K.dx0k[k] = P->dx0k[k];
   }
+  // hybgen.F90:436-452 with qdep==1: dp0cum(1)=0, dp0cum(2)=dp0ij(1)=dp0k(1),
+  // dp0cum(k+1)=dp0cum(k)+dp0k(k); the same additions in the same order.
+  K.dp0cum[0] = 0.0;
+  if (P->kdm >= 1) K.dp0cum[1] = K.dp0cum[0] + P->dp0k[0];
+  for (int k = 2; k <= P->kdm && k < 65; k++) K.dp0cum[k] = K.dp0cum[k - 1] + P->dp0k[k - 1];
   for (int t = 0; t < P->ntracr && t < 64; t++)
     if (P->trcflg[t] == 2) K.trc_t |= 1ull << t;
   return K;

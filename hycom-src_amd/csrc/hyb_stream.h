@@ -1,0 +1,1185 @@
+// hyb_stream.h -- FP64 hybgen column kernels with rolled k-loops and small
+// register windows: no per-lane arrays, one code path for any kdm <= 64.
+//
+// Every floating-point expression keeps the reference's operands and
+// operation order; only the loop structure differs from hybgen.F90.
+//
+// regrid_stream (hybgenaj, hybgen.F90:399-1529) makes three forward passes
+// over a column:
+//   1. interfaces p, fixed-layer count, relaxation prefix and kp
+//      (399-598); p goes to the per-column scratch `pres`;
+//   2. overturn / unmix of layer kp (606-970), in place;
+//   3. interface relocation (1065-1494) on a 7-deep window p(k-3..k+3),
+//      then, level by level as each p(k-2) becomes final, the maximum
+//      thickness pass (1496-1509) and the rebuild of dp and p (1523-1529).
+//   The two reference loops that write below the window -- the layer-1
+//   thickness loop (1051-1063) and the fixed-layer order loop (1098-1107)
+//   -- raise a monotone run of p to a floor, so they are applied as levels
+//   enter the window.  dp0ij and qhrlx of level x are recomputed from the
+//   initial p(x) when level x enters the window.
+//
+// remap_stream (1530-1591 with the hybgen_*_remap/_coefs routines) walks
+// the old layers l = 1..kk once for up to kNFMax fields at a time (T and S
+// first, with th3d), carrying each field's values s(l-1..l+2) and
+// reconstruction state in registers.  The per-layer metrics (1/(dp+dp)
+// etc.) and the per-output weights are shared by the fields.  Outputs are
+// written in place as they complete; regrid_stream flags the columns where
+// an output would overwrite a level the walk has not read yet, and those
+// columns go to remap_fallback.  The clamp to the column's [min,max]
+// (e.g. 2836-2837) is exact for an output inside the range of the levels
+// read so far; other outputs are stored unclamped, marked, and clamped
+// once the whole column has been read.
+#pragma once
+
+#include "hyb_column.h"
+#include "hyb_fast.h"
+
+namespace hyb {
+
+constexpr int32_t kUnsafe = 1 << 16;  // kp word: remap this column in remap_fallback
+constexpr int kNFMax = 6;             // fields per remap walk
+constexpr int kKMax = 64;             // largest kdm
+
+// minimum thickness of level k (hybgen.F90:456-466) from the initial
+// interface pk = p(k) and dp0cum(k), qdep == 1; *qh = qhrlx(k+1).
+HYB_HD double dp0ij_qh(const KParams& P, double d, double pk, double dp0cum_k, double* qh) {
+  double q, qts;
+  if (d <= P.dp00i) {
+    q = d;
+    qts = 0.0;
+  } else {
+    q = dmax(P.dp00i, d * d / dmax(d, pk - dp0cum_k));
+    qts = 1.0 - (q - P.dp00i) / (d - P.dp00i);
+  }
+  *qh = 1.0 / (1.0 + qts * (P.hybrlx - 1.0));
+  return dmin(q, d);
+}
+HYB_HD double dp0ij_only(const KParams& P, double d, double pk, double dp0cum_k) {
+  double q;
+  if (d <= P.dp00i) {
+    q = d;
+  } else {
+    q = dmax(P.dp00i, d * d / dmax(d, pk - dp0cum_k));
+  }
+  return dmin(q, d);
+}
+
+template <int W>
+HYB_HD void shl(double (&a)[W], double v) {
+#pragma unroll
+  for (int q = 0; q < W - 1; q++) a[q] = a[q + 1];
+  a[W - 1] = v;
+}
+
+// ---------------------------------------------------------------------
+// regrid of one p-column (hybgenaj up to the remap)
+// ---------------------------------------------------------------------
+HYB_HD void regrid_stream(const KParams& P, const View& V, const Scratch& W, int i, int j,
+                          int n) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int ntracr = P.ntracr;
+  double* temp = V.temp + c0 + lev4(V, 1, n);
+  double* saln = V.saln + c0 + lev4(V, 1, n);
+  double* th3d = V.th3d + c0 + lev4(V, 1, n);
+  double* dpc = V.dp + c0 + lev4(V, 1, n);
+  double* pres = W.pres + c0;
+  const double* theta = V.theta + c0;
+#define FK(x, k) (x)[(int64_t)((k) - 1) * S]
+#define TRK(t, k) V.tracer[c0 + levt(V, k, n, t)]
+  const double qqmn = -4.0, qqmx = 2.0;
+  const double cusha = qqmn * qqmn * (qqmx - 1.0) / ((qqmx - qqmn) * (qqmx - qqmn));
+  const double cushb = 1.0 / qqmn;
+  auto cushn = [&](double delp, double dp0) -> double {
+    double qq = dmax(qqmn, dmin(qqmx, delp / dp0));
+    double a = 1.0 - cushb * qq;
+    return dp0 * (1.0 + cusha * (a * a)) * dmax(1.0, delp / (dp0 * qqmx));
+  };
+  const double dpthin = kDpthin;
+  const double rthin = P.hybthn;
+  const double topiso = V.topiso[c0];
+
+  // ------------------------------------------------ pass 1: 399-598
+  double qdep;
+  if (P.dpns == P.dsns) {
+    qdep = 1.0;
+  } else {
+    qdep = dmax(0.0, dmin(1.0, (V.depths[c0] - P.dsns) / (P.dpns - P.dsns)));
+  }
+  const bool tf = qdep < 1.0;
+  const double dp0ij1 = tf ? qdep * P.dp0k[0] + (1.0 - qdep) * P.ds0k[0] : P.dp0k[0];
+  double pk = 0.0;               // p(k)
+  double cumk = 0.0 + dp0ij1;    // dp0cum(k), k = 2
+  double pk1 = pk + FK(dpc, 1);  // p(k+1), k = 1
+  pres[0] = 0.0;
+  pres[S] = pk1;
+  int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
+  int kp = 2;
+#pragma unroll 1
+  for (int k = 2; k <= K; k++) {
+    pk = pk1;
+    pk1 = pk + FK(dpc, k);
+    pres[(int64_t)k * S] = pk1;
+    if (k >= 3 && pk1 - pk >= dpthin) kp = k;  // 590-598: the deepest such k
+    double dtf = 0.0, cumk1;
+    if (tf) {
+      dtf = qdep * P.dp0k[k - 1] + (1.0 - qdep) * P.ds0k[k - 1];
+      cumk1 = cumk + dtf;
+    } else {
+      cumk1 = P.dp0cum[k];
+    }
+    if (phase == 1) {  // 479-488
+      if (cumk >= topiso) {
+        phase = 2;
+        fixall = fixlay;
+      } else {
+        fixlay = fixlay + 1;
+        qfix = k + 1;
+      }
+    }
+    if (phase == 2) {  // 550-573
+      const double d0k = tf ? dtf : dp0ij_only(P, P.dp0k[k - 1], pk, cumk);
+      if (pk1 > cumk1 + 0.1 * d0k) {
+        if (fixlay > fixall) {
+          if (pk > cumk) fixlay = fixlay - 1;
+        }
+        phase = 3;
+      } else {
+        if (k > qfix) qfix = k;
+        fixlay = fixlay + 1;
+      }
+    }
+    cumk = cumk1;
+  }
+  const double zx = pk1;  // p(kk+1); the overturn does not move it
+
+  // ------------------------------------------------ pass 2: 606-970
+  // The overturn moves p(kp0) or p(kp0-1) only; dp0ij and qhrlx are
+  // functions of the initial p, so those two values are kept.
+  const int kp0 = kp;
+  const double pin_a = pres[(int64_t)(kp0 - 2) * S];  // p(kp0-1)
+  const double pin_b = pres[(int64_t)(kp0 - 1) * S];  // p(kp0)
+  auto p_init = [&](int x, double cur) -> double {
+    return x == kp0 ? pin_b : (x == kp0 - 1 ? pin_a : cur);
+  };
+  auto qhrlx_of = [&](int x) -> double {  // final qhrlx(x)
+    if (tf || x <= 2 || x <= qfix) return 1.0;
+    double qh;
+    (void)dp0ij_qh(P, P.dp0k[x - 2], p_init(x - 1, pres[(int64_t)(x - 2) * S]),
+                   P.dp0cum[x - 2], &qh);
+    return qh;
+  };
+  {
+    const int k = kp;
+    const int ka = (k - 2 > 1) ? k - 2 : 1;
+    const double pkk = pres[(int64_t)(k - 1) * S], pkm = pres[(int64_t)(k - 2) * S],
+                 pkp = pres[(int64_t)k * S];
+    const double thk = FK(theta, k);
+    const double tk = FK(th3d, k), tkm = FK(th3d, k - 1), tka = FK(th3d, ka);
+    if (k > fixlay + 1 && qdep == 1.0 && pkk - pkm >= dpthin && thk - kEpsil > tk &&
+        tkm > tk && tka > tk) {
+      // 609-686 entrain layer k into k-1
+      double q = (pkp - pkk) / (pkp - pkm);
+      FK(temp, k - 1) = FK(temp, k - 1) - q * (FK(temp, k - 1) - FK(temp, k));
+      FK(saln, k - 1) = FK(saln, k - 1) - q * (FK(saln, k - 1) - FK(saln, k));
+      FK(th3d, k - 1) = sig(FK(temp, k - 1), FK(saln, k - 1)) - P.thbase;
+      for (int t = 1; t <= ntracr; t++)
+        TRK(t, k - 1) = TRK(t, k - 1) - q * (TRK(t, k - 1) - TRK(t, k));
+      pres[(int64_t)(k - 1) * S] = pkp;
+      kp = k - 1;
+    } else if (k > fixlay + 1 && qdep == 1.0 && pkk - pkm >= dpthin && thk - kEpsil > tk &&
+               tkm > tk) {
+      // 687-811 swap with the layer above
+      if (pkp - pkk <= pkk - pkm) {
+        double st = FK(temp, k - 1), ss = FK(saln, k - 1), sh = FK(th3d, k - 1);
+        double q = (pkp - pkk) / (pkk - pkm);
+        FK(temp, k - 1) = FK(temp, k - 1) - q * (FK(temp, k - 1) - FK(temp, k));
+        FK(saln, k - 1) = FK(saln, k - 1) - q * (FK(saln, k - 1) - FK(saln, k));
+        FK(th3d, k - 1) = sig(FK(temp, k - 1), FK(saln, k - 1)) - P.thbase;
+        FK(temp, k) = st;
+        FK(saln, k) = ss;
+        FK(th3d, k) = sh;
+        for (int t = 1; t <= ntracr; t++) {
+          double sv = TRK(t, k - 1);
+          TRK(t, k - 1) = TRK(t, k - 1) - q * (TRK(t, k - 1) - TRK(t, k));
+          TRK(t, k) = sv;
+        }
+      } else {
+        double st = FK(temp, k), ss = FK(saln, k), sh = FK(th3d, k);
+        double q = (pkk - pkm) / (pkp - pkk);
+        FK(temp, k) = FK(temp, k) + q * (FK(temp, k - 1) - FK(temp, k));
+        FK(saln, k) = FK(saln, k) + q * (FK(saln, k - 1) - FK(saln, k));
+        FK(th3d, k) = sig(FK(temp, k), FK(saln, k)) - P.thbase;
+        FK(temp, k - 1) = st;
+        FK(saln, k - 1) = ss;
+        FK(th3d, k - 1) = sh;
+        for (int t = 1; t <= ntracr; t++) {
+          double sv = TRK(t, k);
+          TRK(t, k) = TRK(t, k) + q * (TRK(t, k - 1) - TRK(t, k));
+          TRK(t, k - 1) = sv;
+        }
+      }
+    }
+  }
+  {
+    // 813-970 unmix a too-light deepest layer
+    const int k = kp;
+    const double pkk = pres[(int64_t)(k - 1) * S], pkm = pres[(int64_t)(k - 2) * S],
+                 pkp = pres[(int64_t)k * S];
+    const double dpthck = P.dx0k[k - 2] * 0.95;
+    const double thk = FK(theta, k), thkm = FK(theta, k - 1);
+    const double tk = FK(th3d, k), tkm = FK(th3d, k - 1);
+    if (k > fixlay + 1 && qdep == 1.0 && pkk - pkm >= dpthin && pkk - pkm < dpthck &&
+        thk - kEpsil > tk && thkm < tk && dabs(thkm - tkm) < P.hybiso &&
+        (tk - tkm) > (thk - thkm) * 0.001) {
+      int ka = 1;
+      for (int kt = k - 2; kt >= 2; kt--) {
+        if (tkm - FK(th3d, kt) >= thkm - FK(theta, k - 2)) {
+          ka = kt;
+          break;
+        }
+      }
+      double delsm = dabs(FK(saln, ka) - FK(saln, k - 1));
+      double dels = dabs(FK(saln, k - 1) - FK(saln, k));
+      double deltm = dabs(FK(temp, ka) - FK(temp, k - 1));
+      double delt = dabs(FK(temp, k - 1) - FK(temp, k));
+      double q = dmin3(FK(temp, ka), FK(temp, k - 1), FK(temp, k));
+      if (q > 6.0) {
+        deltm = dmin(deltm, 6.0 * (thk - thkm));
+      } else {
+        deltm = dmin(deltm, 10.0 * (thk - thkm));
+      }
+      delsm = dmin(delsm, 1.3 * (thk - thkm));
+      double qts = 0.0;
+      if (delt > kEpsil) qts = dmax(qts, (dmin(deltm, 2.0 * delt) - delt) / delt);
+      if (dels > kEpsil) qts = dmax(qts, (dmin(delsm, 2.0 * dels) - dels) / dels);
+      q = (thk - tk) / (thk - tkm);
+      q = dmin(q, qts / (1.0 + qts));
+      q = qhrlx_of(k) * q;
+      const double p_hat = q * (pkp - pkk);
+      const double pknew = pkk + p_hat;
+      pres[(int64_t)(k - 1) * S] = pknew;
+      FK(temp, k) = FK(temp, k) + (q / (1.0 - q)) * (FK(temp, k) - FK(temp, k - 1));
+      FK(saln, k) = FK(saln, k) + (q / (1.0 - q)) * (FK(saln, k) - FK(saln, k - 1));
+      FK(th3d, k) = sig(FK(temp, k), FK(saln, k)) - P.thbase;
+      if (ntracr > 0 && p_hat != 0.0) {
+        double qtr = p_hat / dmax(p_hat, pknew - pkm);
+        for (int t = 1; t <= ntracr; t++) {
+          if ((P.trc_t >> (t - 1)) & 1ull) {
+            TRK(t, k) = TRK(t, k) + (q / (1.0 - q)) * (TRK(t, k) - TRK(t, k - 1));
+          } else {
+            TRK(t, k - 1) = TRK(t, k - 1) + qtr * (TRK(t, k) - TRK(t, k - 1));
+          }
+        }
+      }
+    }
+  }
+  // 972-1009 (nhybrd == kk): levels below kp take level kp's values; the
+  // remap applies this fill to T, S and the tracers as it loads them.
+  const double th_kp = FK(th3d, kp);
+  auto th3d_eff = [&](int x) -> double { return x > kp ? th_kp : FK(th3d, x); };
+
+  // ------------------------------------------------ pass 3: 1036-1529
+  const double floor1 = 0.0 + dp0ij1;              // 1053-1055: p(1)+dp0ij(1)
+  const double pbot = zx < floor1 ? floor1 : zx;   // p(kk+1) after 1056-1062
+  double floor2 = 0.0;                             // 1098-1107, once set
+  int floor2_from = K + 2;
+  auto with_floor = [&](int x, double v) -> double {
+    if (x == 2) v = floor1;
+    else if (x >= 3 && v < floor1) v = floor1;
+    if (x >= floor2_from && x <= K && v < floor2) v = floor2;
+    return v;
+  };
+  auto raw_at = [&](int x) -> double {
+    return (x >= 1 && x <= K + 1) ? pres[(int64_t)(x - 1) * S] : 0.0;
+  };
+  double tfc[4] = {0.0, 0.0, 0.0, 0.0};  // dp0cum(k-2..k+1) when qdep < 1
+  auto dp0ij_tf = [&](int x) -> double {
+    return qdep * P.dp0k[x - 1] + (1.0 - qdep) * P.ds0k[x - 1];
+  };
+  auto dp0cum_at = [&](int x, int k) -> double {
+    return tf ? tfc[x - (k - 2)] : P.dp0cum[x - 1];
+  };
+  uint64_t lcm = 0;  // 1036-1045 from pres(x), pres(x+1), th3d(x), theta(x)
+  auto lcm_set = [&](int x, double r0, double r1, double tx, double thx) {
+    if (P.isopcm && x > fixlay && ((r1 - r0) <= dpthin || dabs(tx - thx) < P.hybiso))
+      lcm |= 1ull << (x - 1);
+  };
+  // windows at k = 2
+  double pw[7];          // p(k-3..k+3)
+  double rw[3];          // pres(k+1..k+3) as left by the overturn
+  double tw[4], thw[4];  // th3d (filled) and theta, levels k-1..k+2
+  double dw[5];          // dp0ij(k-3..k+1)
+  double qw[4];          // qhrlx(k-2..k+1)
+  double qh_next = 1.0;  // qhrlx(k+2) before the fixed-layer override
+#pragma unroll
+  for (int q = 0; q < 7; q++) {
+    const int x = q - 1;
+    pw[q] = (x >= 1 && x <= K + 1) ? with_floor(x, raw_at(x)) : 0.0;
+  }
+  rw[0] = raw_at(3);
+  rw[1] = raw_at(4);
+  rw[2] = raw_at(5);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int x = q + 1;
+    tw[q] = (x <= K) ? th3d_eff(x) : 0.0;
+    thw[q] = (x <= K) ? FK(theta, x) : 0.0;
+  }
+  {
+    double r = raw_at(1);
+#pragma unroll
+    for (int x = 1; x <= 4; x++) {
+      const double r1 = raw_at(x + 1);
+      if (x <= K) lcm_set(x, r, r1, tw[x - 1], thw[x - 1]);
+      r = r1;
+    }
+  }
+  dw[0] = 0.0;
+  dw[1] = 0.0;
+  dw[2] = dp0ij1;
+  qw[0] = 1.0;
+  qw[1] = 1.0;
+  qw[2] = 1.0;  // qhrlx(2)
+  if (tf) {
+    tfc[1] = 0.0;              // dp0cum(1)
+    tfc[2] = tfc[1] + dp0ij1;  // dp0cum(2)
+    dw[3] = K >= 2 ? dp0ij_tf(2) : 0.0;
+    tfc[3] = tfc[2] + dw[3];   // dp0cum(3)
+    dw[4] = K >= 3 ? dp0ij_tf(3) : 0.0;
+    qw[3] = 1.0;
+  } else {
+    double qh3 = 1.0;
+    dw[3] = K >= 2 ? dp0ij_qh(P, P.dp0k[1], p_init(2, raw_at(2)), P.dp0cum[1], &qh3) : 0.0;
+    qw[3] = (3 <= qfix) ? 1.0 : qh3;
+    dw[4] = K >= 3 ? dp0ij_qh(P, P.dp0k[2], p_init(3, raw_at(3)), P.dp0cum[2], &qh_next) : 0.0;
+  }
+
+  // 1496-1529 as each interface becomes final, plus the in-place check
+  double pf = 0.0, prs = 0.0;
+  double* pg = V.p + c0;
+  bool unsafe = false;
+  auto consume = [&](int x, double v) {
+    if (x == 1) {
+      pf = v;
+      prs = v;
+      pg[0] = v;
+      return;
+    }
+    const int jx = x - 1;
+    if (jx >= 2 && jx <= K - 1 && (v - pf) > P.dx0k[jx - 1]) {
+      v = pf + P.dx0k[jx - 1];
+      lcm &= ~(3ull << (jx - 2));
+    }
+    const double d = dmax(v - prs, 0.0);
+    FK(dpc, jx) = d;
+    prs = prs + d;
+    pg[(int64_t)jx * S] = prs;
+    pf = v;
+    // remap output jx ends in old layer lb <= jx-3, before the old layer
+    // that reaches the bottom, only if pres(jx-2) >= zb = min(prsf(jx+1),zx)
+    if (jx >= 4) {
+      const double zb = dmin(prs, zx);
+      if (zb < zx && pres[(int64_t)(jx - 3) * S] >= zb) unsafe = true;
+    }
+  };
+
+  double q, p_hat, p_hat0, p_hat2, p_hat3;
+#pragma unroll 1
+  for (int k = 2; k <= K; k++) {
+#define PW(x) pw[(x) - (k - 3)]
+    const double dm1 = dw[2], dm2 = dw[1], dm3 = dw[0], d0 = dw[3], dp1 = dw[4];
+    const double qm1 = qw[1], qm2 = qw[0], q0 = qw[2], qp1 = qw[3];
+    if (k <= fixlay) {
+      // 1092-1108
+      if (k < K) {
+        PW(k + 1) = dmin(dp0cum_at(k + 1, k), pbot);
+        if (k == fixlay) {
+          floor2 = PW(k + 1);
+          floor2_from = k + 2;
+          if (k + 2 <= K && PW(k + 2) < floor2) PW(k + 2) = floor2;
+          if (k + 3 <= K && PW(k + 3) < floor2) PW(k + 3) = floor2;
+        }
+      }
+    } else {
+      const double thk = thw[1], thm = thw[0];
+      const double tk = tw[1], tkm = tw[0];
+      double dp_top = PW(k) - PW(k - 1) + kEpsil;
+      double dp_cen = PW(k + 1) - PW(k) + kEpsil;
+      double dp_bot, dp_far, dp_inc;
+      if (k == K) {
+        dp_bot = dp_cen;
+        dp_far = dp_cen;
+      } else if (k == K - 1) {
+        dp_bot = PW(k + 2) - PW(k + 1) + kEpsil;
+        dp_far = dp_bot;
+      } else {
+        dp_bot = PW(k + 2) - PW(k + 1) + kEpsil;
+        dp_far = PW(k + 3) - PW(k + 2) + kEpsil;
+      }
+      // hybthk <= 0.9 makes the first test false at k == kk and the second
+      // false at k >= kk-1; the guards keep th3d(kk+1..) out of reach.
+      if (k < K && dp_cen < P.hybthk * dmin(dp_top, dp_bot)) {
+        // 1131-1159 thick-thin-thick
+        const double thp = thw[2], tkp = tw[2];
+        if (dmin(dp_top, dp_bot) < 0.2 * dmax(dp_top, dp_bot)) {
+          q = dp_top / (dp_top + dp_bot);
+        } else if (thk > tkm && thk < tkp) {
+          q = (thk - tkp) / (tkm - tkp);
+        } else {
+          q = (thk - thp) / (thm - thp);
+        }
+        if (dp_top <= dp_bot) {
+          dp_inc = (P.hybthk * dp_top - dp_cen) / (1.0 + q * P.hybthk);
+        } else {
+          dp_inc = (P.hybthk * dp_bot - dp_cen) / (1.0 + (1.0 - q) * P.hybthk);
+        }
+        p_hat = PW(k) - q * dp_inc;
+        PW(k) = (1.0 - q0) * PW(k) + q0 * p_hat;
+        p_hat = PW(k + 1) + (1.0 - q) * dp_inc;
+        PW(k + 1) = (1.0 - q0) * PW(k + 1) + q0 * p_hat;
+      } else if (k < K - 1 && dp_cen + dp_bot < P.hybthk * dmin(dp_top, dp_far)) {
+        // 1178-1208 thick-thin-thin-thick
+        const double thp = thw[2], thp2 = thw[3], tkp2 = tw[3];
+        const double thmid = 0.5 * (thk + thp);
+        if (dmin(dp_top, dp_far) < 0.2 * dmax(dp_top, dp_far)) {
+          q = dp_top / (dp_top + dp_far);
+        } else if (thmid > tkm && thmid < tkp2) {
+          q = (thmid - tkp2) / (tkm - tkp2);
+        } else {
+          q = (thmid - thp2) / (thm - thp2);
+        }
+        if (dp_top <= dp_far) {
+          dp_inc = (P.hybthk * dp_top - (dp_cen + dp_bot)) / (1.0 + q * P.hybthk);
+        } else {
+          dp_inc = (P.hybthk * dp_far - (dp_cen + dp_bot)) / (1.0 + (1.0 - q) * P.hybthk);
+        }
+        p_hat = PW(k) - q * dp_inc;
+        PW(k) = (1.0 - q0) * PW(k) + q0 * p_hat;
+        p_hat = PW(k + 2) + (1.0 - q) * dp_inc;
+        PW(k + 2) = (1.0 - q0) * PW(k + 2) + q0 * p_hat;
+      } else if (tk > thk + kEpsil && k > fixlay + 1) {
+        // 1237-1387 layer too dense: move the upper interface up
+        bool ltop;
+        if (rthin * (PW(k) - PW(k - 1)) <= (PW(k + 1) - PW(k))) {
+          ltop = true;
+        } else if (rthin * (PW(k + 1) - PW(k)) <= (PW(k) - PW(k - 1))) {
+          ltop = false;
+        } else {
+          ltop = thm - tkm >= tk - thk;
+        }
+        if (tkm >= thm || !ltop || PW(k) <= dp0cum_at(k, k) + kOnem) {
+          p_hat0 = 0.0;
+          if ((thk - tkm) <= kEpsil) {
+            p_hat = PW(k - 1) + dm1;
+          } else {
+            q = (thk - tk) / (thk - tkm);
+            p_hat0 = PW(k) + q * (PW(k + 1) - PW(k));
+            if (k == fixlay + 2) {
+              p_hat = PW(k - 1) + dmax(p_hat0 - PW(k - 1), dm1);
+            } else {
+              p_hat = PW(k - 1) + cushn(p_hat0 - PW(k - 1), dm1);
+            }
+          }
+          p_hat = dmin(p_hat, pbot);
+          if (k <= fixlay + 2) {
+            // do nothing
+          } else if (p_hat >= PW(k) && PW(k - 1) > dp0cum_at(k - 1, k) + kTenm &&
+                     (pbot - PW(k - 1) < P.thkbop || PW(k - 1) - PW(k - 2) > qqmx * dm2)) {
+            if (k == fixlay + 3) {
+              p_hat2 = PW(k - 2) + dmax(PW(k - 1) - p_hat + p_hat0 - PW(k - 2), dm2);
+            } else {
+              p_hat2 = PW(k - 2) + cushn(PW(k - 1) - p_hat + p_hat0 - PW(k - 2), dm2);
+            }
+            if (p_hat2 < PW(k - 1) - kOnemm) {
+              PW(k - 1) = (1.0 - qm1) * PW(k - 1) + qm1 * dmax(p_hat2, 2.0 * PW(k - 1) - p_hat);
+              p_hat = PW(k - 1) + cushn(p_hat0 - PW(k - 1), dm1);
+            } else if (k <= fixlay + 3) {
+              // do nothing
+            } else if (PW(k - 2) > dp0cum_at(k - 2, k) + kTenm &&
+                       (pbot - PW(k - 2) < P.thkbop || PW(k - 2) - PW(k - 3) > qqmx * dm3)) {
+              if (k == fixlay + 4) {
+                p_hat3 = PW(k - 3) + dmax(PW(k - 2) - p_hat + p_hat0 - PW(k - 3), dm3);
+              } else {
+                p_hat3 = PW(k - 3) + cushn(PW(k - 2) - p_hat + p_hat0 - PW(k - 3), dm3);
+              }
+              if (p_hat3 < PW(k - 2) - kOnemm) {
+                PW(k - 2) =
+                    (1.0 - qm2) * PW(k - 2) + qm2 * dmax(p_hat3, 2.0 * PW(k - 2) - PW(k - 1));
+                p_hat2 = PW(k - 2) + cushn(PW(k - 1) - p_hat + p_hat0 - PW(k - 2), dm2);
+                if (p_hat2 < PW(k - 1) - kOnemm) {
+                  PW(k - 1) =
+                      (1.0 - qm1) * PW(k - 1) + qm1 * dmax(p_hat2, 2.0 * PW(k - 1) - p_hat);
+                  p_hat = PW(k - 1) + cushn(p_hat0 - PW(k - 1), dm1);
+                }
+              }
+            }
+          }
+          if (p_hat < PW(k)) {
+            PW(k) = (1.0 - q0) * PW(k) + q0 * p_hat;
+          }
+        }
+      } else if (tk < thk - kEpsil) {
+        // 1389-1475 layer too light: move the lower interface down
+        if (PW(k + 1) < pbot) {
+          const double thp = thw[2], tkp = tw[2];
+          bool ltop;
+          if (rthin * (PW(k + 1) - PW(k)) <= (PW(k + 2) - PW(k + 1))) {
+            ltop = true;
+          } else if (rthin * (PW(k + 2) - PW(k + 1)) <= (PW(k + 1) - PW(k))) {
+            ltop = false;
+          } else {
+            ltop = thk - tk >= tkp - thp;
+          }
+          if (tkp <= thp || ltop || PW(k + 1) <= dp0cum_at(k + 1, k) + kOnem) {
+            if ((tkp - thk) <= kEpsil) {
+              p_hat = PW(k + 2);
+            } else {
+              q = (tk - thk) / (tkp - thk);
+              p_hat = PW(k + 1) + q * (PW(k) - PW(k + 1));
+            }
+            const double p3 = (k + 3 <= K + 1) ? PW(k + 3) : pbot;
+            if (p3 < pbot) {
+              if (pbot - PW(k) > d0 + dp1) {
+                p_hat = PW(k + 2) - cushn(PW(k + 2) - p_hat, dp1);
+              }
+              p_hat = PW(k) + dmax(p_hat - PW(k), d0);
+              p_hat = dmin(p_hat, dmax(0.5 * (PW(k + 1) + PW(k + 2)), PW(k + 2) - dp1));
+            } else {
+              p_hat = dmin(PW(k + 2), p_hat);
+            }
+            if (p_hat > PW(k + 1)) {
+              PW(k + 1) = (1.0 - qp1) * PW(k + 1) + qp1 * p_hat;
+            }
+          }
+        }
+      }
+      // 1479-1490 layer above still too thin
+      p_hat0 = dmin(PW(k - 1) + dm1, pbot);
+      if (p_hat0 > PW(k)) {
+        p_hat = (1.0 - qm1) * PW(k) + qm1 * p_hat0;
+        PW(k) = dmin(p_hat, PW(k + 1));
+      }
+    }
+#undef PW
+    // p(k-2) is final now
+    if (k >= 3) consume(k - 2, pw[1]);
+    // slide: level k+4 enters pw, k+3 enters tw/thw/lcm, k+2 enters dw/qw
+    const double r4 = raw_at(k + 4);
+    shl(pw, (k + 4 <= K + 1) ? with_floor(k + 4, r4) : 0.0);
+    const double th3 = (k + 3 <= K) ? th3d_eff(k + 3) : 0.0;
+    const double ta3 = (k + 3 <= K) ? FK(theta, k + 3) : 0.0;
+    if (k + 3 <= K) lcm_set(k + 3, rw[2], r4, th3, ta3);
+    shl(tw, th3);
+    shl(thw, ta3);
+    if (tf) {
+      const double dn = (k + 2 <= K) ? dp0ij_tf(k + 2) : 0.0;
+      shl(tfc, tfc[3] + dw[4]);  // dp0cum(k+2) = dp0cum(k+1) + dp0ij(k+1)
+      shl(dw, dn);
+      shl(qw, 1.0);
+    } else {
+      shl(qw, (k + 2 <= qfix) ? 1.0 : qh_next);
+      double qh = 1.0;
+      const double dn =
+          (k + 2 <= K) ? dp0ij_qh(P, P.dp0k[k + 1], p_init(k + 2, rw[1]), P.dp0cum[k + 1], &qh)
+                       : 0.0;
+      qh_next = qh;
+      shl(dw, dn);
+    }
+    shl(rw, r4);
+  }
+  // levels kk-1, kk, kk+1 (pw holds p(kk-2..kk+4) now)
+  consume(K - 1, pw[1]);
+  consume(K, pw[2]);
+  consume(K + 1, pw[3]);
+  W.kp[c0] = kp | (unsafe ? kUnsafe : 0);
+  W.lcm[c0] = lcm;
+  if (V.rec_fixlay) {
+    int64_t r = (int64_t)(i - 1) + (int64_t)V.rec_ld * (j - 1);
+    V.rec_fixlay[r] = fixlay;
+    V.rec_kp[r] = kp;
+  }
+#undef FK
+#undef TRK
+}
+
+// ---------------------------------------------------------------------
+// remap of up to kNFMax fields of one column in one walk over the old
+// layers.  po(k) = prsf(k), the new interfaces (k = 1..kk+1).
+// ---------------------------------------------------------------------
+template <int METHOD, class PO>
+HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, double zx,
+                        const PO& po, double* const* fp, int nf, double* th3d, int kp,
+                        uint64_t lcm) {
+  constexpr int NF = kNFMax;
+  const double thin = kDpthin;
+  const double dsmll = 1.0e-8;
+  auto pi_at = [&](int x) -> double {
+    return (x >= 1 && x <= K + 1) ? pres[(int64_t)(x - 1) * S] : 0.0;
+  };
+  auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
+
+  // per-field state
+  double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1), s(l), s(l+1), s(l+2)
+  double smin[NF], smax[NF], skp[NF], sfirst[NF];
+  double acc[NF], ov[NF], prev[NF];
+  double asc[NF], alc[NF];                     // PPM: as(l), raw al(l)
+  double wc1[NF], wc2[NF], wz1[NF], wz2[NF];   // WENO first sweep of layer l
+  double zw3l[NF];                             // WENO interface value zw3(l)
+  // s(x) with the massless fill (972-996) and the running min/max
+  // (e.g. 2727-2731: si_min/si_max over k = 1..kk in order)
+  auto load = [&](int f, int x) -> double {
+    double v;
+    if (x <= kp) {
+      v = fp[f][(int64_t)(x - 1) * S];
+    } else {
+      v = skp[f];
+    }
+    if (x == kp) skp[f] = v;
+    if (x == 1) {
+      smin[f] = v;
+      smax[f] = v;
+    } else {
+      smin[f] = dmin(smin[f], v);
+      smax[f] = dmax(smax[f], v);
+    }
+    return v;
+  };
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    s0[f] = s1[f] = s2[f] = s3[f] = 0.0;
+    smin[f] = smax[f] = skp[f] = sfirst[f] = 0.0;
+    acc[f] = ov[f] = prev[f] = 0.0;
+    asc[f] = alc[f] = 0.0;
+    wc1[f] = wc2[f] = wz1[f] = wz2[f] = zw3l[f] = 0.0;
+    if (f < nf) {
+      s1[f] = load(f, 1);
+      if (K >= 2) s2[f] = load(f, 2);
+      if (K >= 3) s3[f] = load(f, 3);
+      sfirst[f] = s1[f];
+      if (METHOD == 2) alc[f] = s1[f];  // al(1) = s(1)
+      if (METHOD == 3) {                // first sweep of layer 1
+        wc1[f] = s1[f];
+        wc2[f] = s1[f];
+      }
+    }
+  }
+  // interfaces pi(l-1..l+3) at l = 1
+  double pv0 = 0.0, pv1 = pi_at(1), pv2 = pi_at(2), pv3 = pi_at(3), pv4 = pi_at(4);
+  // carried metrics: PPM qdpjp(l); WENO 1/(dp(l)+dp(l+1))
+  double qdpjp_l = 0.0, wq_l = 0.0;
+  if (K >= 2) {
+    const double d1 = dmax(pv2 - pv1, thin), d2 = dmax(pv3 - pv2, thin);
+    qdpjp_l = 1.0 / (d1 + d2);
+    wq_l = 1.0 / (d1 + d2);
+  }
+
+  // walk state (2729-2735)
+  const double zb0 = dmax(po(1), pi_at(1));
+  int lt_c = 1;
+  while (pi_at(lt_c + 1) < zb0 && lt_c < K) lt_c++;
+  double zt_c = zb0;
+  int kc = 1;
+  bool accon = false, acc_thin = false;
+  int lb_act = 0;
+  double zt_act = 0.0, zb_act = 0.0;
+  uint64_t fix = 0;  // outputs stored unclamped
+
+  // store output kc; th3d when T and S are in this group
+  auto emit = [&](const double* v) {
+    bool certain = true;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+      if (f < nf) {
+        fp[f][(int64_t)(kc - 1) * S] = v[f];
+        prev[f] = v[f];
+        if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
+      }
+    if (!certain) {
+      fix |= 1ull << (kc - 1);
+    } else if (th3d) {
+      th3d[(int64_t)(kc - 1) * S] = sig(v[0], v[1]) - P.thbase;
+    }
+    kc = kc + 1;
+  };
+
+#pragma unroll 1
+  for (int l = 1; l <= K; l++) {
+    const double dprs_m = pv1 - pv0, dprs_0 = pv2 - pv1, dprs_p = pv3 - pv2,
+                 dprs_p2 = pv4 - pv3;
+    const double dpi_m = dmax(dprs_m, thin), dpi_0 = dmax(dprs_0, thin),
+                 dpi_p = dmax(dprs_p, thin), dpi_p2 = dmax(dprs_p2, thin);
+    const bool lc_l = lcmb(l);
+    // ------------------------------------------ coefficients of layer l
+    double c1l[NF], c2l[NF], c3l[NF];
+    if (METHOD == 1) {
+      // hybgen_plm_coefs (2393-2458) on dprs
+      const bool flat = (l == 1 || l == K || lc_l || dprs_0 <= thin);
+      double qcen = 0.0;
+      if (!flat) qcen = dprs_0 / (dprs_0 + 0.5 * (dprs_m + dprs_p));
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        c1l[f] = 0.0;
+        c2l[f] = 0.0;
+        c3l[f] = 0.0;
+        if (f < nf && !flat) {
+          double ztop = 2.0 * (s1[f] - s0[f]);
+          double zbot = 2.0 * (s2[f] - s1[f]);
+          double zcen = qcen * (s2[f] - s0[f]);
+          if (ztop * zbot > 0.0) c1l[f] = dsign(dmin3(dabs(zcen), dabs(zbot), dabs(ztop)), zbot);
+        }
+      }
+    } else if (METHOD == 2) {
+      // hybgen_ppm_coefs (2587-2706) on dpi: as(l+1), al(l+1), then the
+      // limited edge values of layer l
+      const int jn = l + 1;
+      const bool do_as = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
+      const bool do_al = jn >= 3 && jn <= K - 1;
+      double qdpjp_n = 0.0, dpq3 = 0.0, dp2jp_l = 0.0, dpj2p_n = 0.0;
+      double dpjp_lm = 0.0, dpjp_n = 0.0, qdp2jp_l = 0.0, qdpj2p_l = 0.0, qdp4 = 0.0;
+      if (jn <= K - 1) {
+        const double dpjp_l = dpi_0 + dpi_p;
+        dpjp_n = dpi_p + dpi_p2;
+        qdpjp_n = 1.0 / dpjp_n;  // qdpjp(l+1)
+        dp2jp_l = dpi_0 + dpjp_l;
+        dpj2p_n = dpjp_n + dpi_p2;
+        dpq3 = dpi_p / (dpi_0 + dpjp_n);
+        if (do_al) {
+          dpjp_lm = dpi_m + dpi_0;
+          const double dpj2p_l = dpjp_l + dpi_p;
+          qdp2jp_l = 1.0 / dp2jp_l;
+          qdpj2p_l = 1.0 / dpj2p_l;
+          qdp4 = 1.0 / (dpjp_lm + dpjp_n);
+        }
+      }
+      const bool lim = l >= 2 && l <= K - 1;
+      const bool lim_flat = lim && (lc_l || dpi_0 <= thin);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        c1l[f] = 0.0;
+        c2l[f] = 0.0;
+        c3l[f] = 0.0;
+        if (f < nf) {
+          double as_n = 0.0, al_n = 0.0;
+          if (do_as) {
+            const double slj = s2[f] - s1[f];
+            const double srj = s3[f] - s2[f];
+            if (slj * srj > 0.0) {
+              double scj = dpq3 * (dp2jp_l * srj * qdpjp_n + dpj2p_n * slj * qdpjp_l);
+              as_n = dsign(dmin3(dabs(2.0 * slj), dabs(scj), dabs(2.0 * srj)), scj);
+            }
+          }
+          if (jn == K) {
+            al_n = s2[f];  // al(kk) = s(kk)
+          } else if (jn == 2) {
+            al_n = s1[f];  // al(2) = s(1)
+          } else if (do_al) {
+            al_n = s1[f] + dpi_0 * (s2[f] - s1[f]) * qdpjp_l +
+                   qdp4 * (2.0 * dpi_p * dpi_0 * qdpjp_l * (s2[f] - s1[f]) *
+                               (dpjp_lm * qdp2jp_l - dpjp_n * qdpj2p_l) -
+                           dpi_0 * as_n * dpjp_lm * qdp2jp_l + dpi_p * asc[f] * dpjp_n * qdpj2p_l);
+          }
+          double al = alc[f], ar;
+          if (l == K - 1) ar = s2[f];  // ar(kk-1) = s(kk)
+          else if (l == K) ar = s1[f];  // ar(kk) = s(kk)
+          else if (l == 1) ar = s1[f];  // ar(1) = s(1)
+          else ar = al_n;               // ar(l) = al(l+1)
+          if (lim) {
+            if (lim_flat) {
+              al = s1[f];
+              ar = s1[f];
+            } else if ((s2[f] - s1[f]) * (s1[f] - s0[f]) <= 0.0) {
+              al = s1[f];
+              ar = s1[f];
+            } else {
+              const double da = ar - al;
+              const double a6 = 6.0 * s1[f] - 3.0 * (al + ar);
+              if (da * a6 > da * da) {
+                al = 3.0 * s1[f] - 2.0 * ar;
+              } else if (da * a6 < -(da * da)) {
+                ar = 3.0 * s1[f] - 2.0 * al;
+              }
+            }
+          }
+          if (al != ar) {
+            c1l[f] = al;
+            c2l[f] = ar - al;
+            c3l[f] = 6.0 * s1[f] - 3.0 * (al + ar);
+          } else {
+            c1l[f] = al;
+          }
+          asc[f] = as_n;
+          alc[f] = al_n;
+        }
+      }
+      qdpjp_l = qdpjp_n;
+    } else if (METHOD == 3) {
+      // hybgen_weno_coefs (2845-2968) on dpi: first sweep of layer l+1,
+      // interface value zw3(l+1), second sweep of layer l
+      const int jn = l + 1;
+      const bool first_full = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
+      double wq_n = 0.0, qmjp = 0.0, dpjm2jp = 0.0;
+      if (jn <= K - 1) wq_n = 1.0 / (dpi_p + dpi_p2);  // qdpjm(l+2)
+      if (first_full) {
+        qmjp = 1.0 / (dpi_0 + dpi_p + dpi_p2);
+        dpjm2jp = dpi_0 + 2.0 * dpi_p + dpi_p2;
+      }
+      const bool second = l >= 2 && l <= K - 1 && !(lc_l || dpi_0 <= thin);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        c1l[f] = 0.0;
+        c2l[f] = 0.0;
+        c3l[f] = 0.0;
+        if (f < nf) {
+          double n1 = s2[f], n2 = s2[f], nz1 = 0.0, nz2 = 0.0;
+          if (first_full) {
+            const double zw3_j = wq_l * (s2[f] - s1[f]);
+            const double zw3_jp = wq_n * (s3[f] - s2[f]);
+            double q001 = dpi_p * zw3_jp;
+            double q002 = dpi_p * zw3_j;
+            if (q001 * q002 < 0.0) {
+              q001 = 0.0;
+              q002 = 0.0;
+            }
+            const double q01 = dpjm2jp * zw3_jp;
+            const double q02 = dpjm2jp * zw3_j;
+            if (dabs(q001) > dabs(q02)) q001 = q02;
+            if (dabs(q002) > dabs(q01)) q002 = q01;
+            const double qq = (q001 - q002) * qmjp;
+            q001 = q001 - qq * dpi_p2;
+            q002 = q002 + qq * dpi_0;
+            n2 = s2[f] + q001;
+            n1 = s2[f] - q002;
+            const double a = 2.0 * q001 - q002, b = 2.0 * q002 - q001;
+            nz1 = a * a;
+            nz2 = b * b;
+          }
+          double zw3n = 0.0;
+          if (jn <= K) {
+            const double q002 = dmax(wz2[f], dsmll);
+            const double q001 = dmax(nz1, dsmll);
+            zw3n = (q001 * wc2[f] + q002 * n1) / (q001 + q002);
+          }
+          c1l[f] = wc1[f];
+          c2l[f] = wc2[f];
+          if (second) {
+            double q01 = zw3n - s1[f];
+            double q02 = s1[f] - zw3l[f];
+            const double q001 = 2.0 * q01;
+            const double q002 = 2.0 * q02;
+            if (q01 * q02 < 0.0) {
+              q01 = 0.0;
+              q02 = 0.0;
+            } else if (dabs(q01) > dabs(q002)) {
+              q01 = q002;
+            } else if (dabs(q02) > dabs(q001)) {
+              q02 = q001;
+            }
+            c1l[f] = s1[f] - q02;
+            c2l[f] = s1[f] + q01;
+          }
+          zw3l[f] = zw3n;
+          wc1[f] = n1;
+          wc2[f] = n2;
+          wz1[f] = nz1;
+          wz2[f] = nz2;
+        }
+      }
+      wq_l = wq_n;
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; f++) c1l[f] = c2l[f] = c3l[f] = 0.0;
+    }
+
+    const double pil = pv1, pil1 = pv2, dprsl = dprs_0;
+    // ------------------------------------------ output in progress
+    if (accon) {
+      if (lb_act == l) {
+        double v[NF];
+        if (acc_thin) {
+#pragma unroll
+          for (int f = 0; f < NF; f++) v[f] = (kc != 1) ? prev[f] : sfirst[f];
+        } else {
+          const double xb = (zb_act - pil) / dmax(dprsl, thin);
+          const double dpb = zb_act - pil;
+          double qb0 = 0.0, qb1 = 0.0, qb2 = 0.0;
+          if (METHOD == 1) {
+            qb0 = xb;
+            qb1 = xb * xb * 0.5;
+          } else if (METHOD == 2) {
+            qb0 = xb;
+            qb1 = xb * xb * 0.5;
+            qb2 = xb * xb * xb / 3.0;
+          } else if (METHOD == 3) {
+            qb1 = (xb - 1.0) * (xb - 1.0);
+            qb2 = qb1 - 1.0 + xb;
+            qb0 = 1.0 - qb1 - qb2;
+          }
+          const double dz = zb_act - zt_act;
+#pragma unroll
+          for (int f = 0; f < NF; f++) {
+            v[f] = 0.0;
+            if (f < nf) {
+              double a = acc[f];
+              const double o = ov[f];
+              if (METHOD == 0) {
+                a = a + dpb * (s1[f] - o);
+              } else if (METHOD == 1) {
+                const double cc0 = s1[f] - o - 0.5 * c1l[f];
+                a = a + dprsl * (cc0 * qb0 + c1l[f] * qb1);
+              } else if (METHOD == 2) {
+                a = a + dprsl * ((c1l[f] - o) * qb0 + (c2l[f] + c3l[f]) * qb1 - c3l[f] * qb2);
+              } else {
+                a = a + dpb * (qb0 * (s1[f] - o) + qb1 * (c1l[f] - o) + qb2 * (c2l[f] - o));
+              }
+              v[f] = o + a / dz;
+            }
+          }
+        }
+        emit(v);
+        lt_c = l;
+        zt_c = zb_act;
+        accon = false;
+      } else if (!acc_thin) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+          if (f < nf) acc[f] = acc[f] + dprsl * (s1[f] - ov[f]);
+      }
+    }
+    // ------------------------------------------ outputs starting in layer l
+    if (!accon && lt_c == l) {
+      while (kc <= K) {
+        const double zt = zt_c;
+        const double zb = dmin(po(kc + 1), zx);
+        const bool ends_here = (l == K) || (pil1 >= zb);
+        const bool thn = (zb - zt <= thin) || (zt >= zx);
+        if (ends_here) {
+          double v[NF];
+          if (thn) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) v[f] = (kc != 1) ? prev[f] : sfirst[f];
+          } else if (METHOD == 0) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) v[f] = s1[f];
+          } else {
+            const double xt = (zt - pil) / dmax(dprsl, thin);
+            const double xb = (zb - pil) / dmax(dprsl, thin);
+            const double dz = zb - zt;
+            double qt0 = 0.0, qt1 = 0.0, qt2 = 0.0;
+            if (METHOD == 1) {
+              qt1 = (xb * xb - xt * xt - (xb - xt)) * 0.5;
+            } else if (METHOD == 2) {
+              qt0 = (xb - xt);
+              qt1 = (xb * xb - xt * xt) * 0.5;
+              qt2 = (xb * xb * xb - xt * xt * xt) / 3.0;
+            } else {
+              qt1 = xb * xb + xt * xt + xb * xt + 1.0 - 2.0 * (xb + xt);
+              qt2 = qt1 - 1.0 + (xb + xt);
+            }
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+              v[f] = 0.0;
+              if (f < nf) {
+                if (METHOD == 1) {
+                  const double sz = dprsl * (c1l[f] * qt1);
+                  v[f] = s1[f] + sz / dz;
+                } else if (METHOD == 2) {
+                  const double oo = s1[f];
+                  const double sz =
+                      dprsl * ((c1l[f] - oo) * qt0 + (c2l[f] + c3l[f]) * qt1 - c3l[f] * qt2);
+                  v[f] = oo + sz / dz;
+                } else {
+                  const double oo = s1[f];
+                  const double sz = qt1 * (c1l[f] - oo) + qt2 * (c2l[f] - oo);
+                  v[f] = oo + sz;
+                }
+              }
+            }
+          }
+          zt_c = zb;
+          emit(v);
+        } else {
+          // output kc starts in layer l and ends in the first deeper layer
+          // lb with pi(lb+1) >= zb (or kk)
+          int lb;
+          if (l + 1 == K || pv3 >= zb) {
+            lb = l + 1;
+          } else if (l + 2 == K || pv4 >= zb) {
+            lb = l + 2;
+          } else {
+            lb = l + 3;
+            while (lb < K && pi_at(lb + 1) < zb) lb++;
+          }
+          lb_act = lb;
+          zt_act = zt;
+          zb_act = zb;
+          acc_thin = thn;
+          accon = true;
+          if (!thn) {
+            const int mid = (l + lb) / 2;
+            const double xt = (zt - pil) / dmax(dprsl, thin);
+            const double dpt = pil1 - zt;
+            double qt0 = 0.0, qt1 = 0.0, qt2 = 0.0;
+            if (METHOD == 1) {
+              qt0 = (1.0 - xt);
+              qt1 = (1.0 - xt * xt) * 0.5;
+            } else if (METHOD == 2) {
+              qt0 = (1.0 - xt);
+              qt1 = (1.0 - xt * xt) * 0.5;
+              qt2 = (1.0 - xt * xt * xt) / 3.0;
+            } else if (METHOD == 3) {
+              qt1 = xt * (xt - 1.0);
+              qt2 = qt1 + xt;
+              qt0 = 1.0 - qt1 - qt2;
+            }
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+              if (f < nf) {
+                // o = s(mid), with the fill; levels > l+2 are not written yet
+                double o;
+                if (mid == l) o = s1[f];
+                else if (mid == l + 1) o = s2[f];
+                else if (mid == l + 2) o = s3[f];
+                else if (kp <= l + 2) o = skp[f];
+                else o = fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S];
+                ov[f] = o;
+                if (METHOD == 0) {
+                  acc[f] = dpt * (s1[f] - o);
+                } else if (METHOD == 1) {
+                  const double cc0 = s1[f] - o - 0.5 * c1l[f];
+                  acc[f] = dprsl * (cc0 * qt0 + c1l[f] * qt1);
+                } else if (METHOD == 2) {
+                  acc[f] = dprsl * ((c1l[f] - o) * qt0 + (c2l[f] + c3l[f]) * qt1 - c3l[f] * qt2);
+                } else {
+                  acc[f] = dpt * (qt0 * (s1[f] - o) + qt1 * (c1l[f] - o) + qt2 * (c2l[f] - o));
+                }
+              }
+            }
+          }
+          break;
+        }
+      }
+    }
+    if (kc > K) break;
+    // ------------------------------------------ slide to layer l+1
+    pv0 = pv1;
+    pv1 = pv2;
+    pv2 = pv3;
+    pv3 = pv4;
+    pv4 = pi_at(l + 4);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      s0[f] = s1[f];
+      s1[f] = s2[f];
+      s2[f] = s3[f];
+      s3[f] = 0.0;
+      if (f < nf && l + 3 <= K) s3[f] = load(f, l + 3);
+    }
+  }
+  // clamp of the outputs stored unclamped (running range -> column range)
+  while (fix) {
+    const int k = __builtin_ctzll(fix) + 1;
+    fix &= fix - 1;
+    double v[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      v[f] = 0.0;
+      if (f < nf) {
+        double x = fp[f][(int64_t)(k - 1) * S];
+        x = dmax(smin[f], x);
+        x = dmin(smax[f], x);
+        fp[f][(int64_t)(k - 1) * S] = x;
+        v[f] = x;
+      }
+    }
+    if (th3d) th3d[(int64_t)(k - 1) * S] = sig(v[0], v[1]) - P.thbase;
+  }
+}
+
+template <int METHOD, class PO>
+HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
+                         const PO& po) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int kpw = W.kp[c0];
+  if (kpw & kUnsafe) return;
+  const int kp = kpw & 0xffff;
+  const uint64_t lcm = W.lcm[c0];
+  const double* pres = W.pres + c0;
+  const double zx = pres[(int64_t)K * S];
+  const int nfields = 2 + P.ntracr;
+  for (int g0 = 0; g0 < nfields; g0 += kNFMax) {
+    const int nf = (nfields - g0 < kNFMax) ? nfields - g0 : kNFMax;
+    double* fp[kNFMax];
+#pragma unroll
+    for (int f = 0; f < kNFMax; f++) {
+      const int g = g0 + f;
+      if (f >= nf) fp[f] = nullptr;
+      else if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
+      else if (g == 1) fp[f] = V.saln + c0 + lev4(V, 1, n);
+      else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
+    }
+    double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
+    walk_stream<METHOD>(P, pres, S, K, zx, po, fp, nf, th, kp, lcm);
+  }
+}
+
+// ---------------------------------------------------------------------
+// Remap of a column flagged kUnsafe: copies the column first (1013-1034),
+// then the reference-shaped coefs/walk of hyb_column.h.
+// ---------------------------------------------------------------------
+HYB_HD void remap_fallback(const KParams& P, const View& V, const Scratch& W, int i, int j,
+                           int n) {
+  const int kk = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int kpw = W.kp[c0];
+  if (!(kpw & kUnsafe)) return;
+  const int kp = kpw & 0xffff;
+  const uint64_t lcm = W.lcm[c0];
+  const double dpthin = kDpthin;
+  double pres[kKMax + 2], dprs[kKMax + 2], dpi[kKMax + 2], prsf[kKMax + 2];
+  double sfld[kKMax + 2], c1[kKMax + 2], c2[kKMax + 2], c3[2 * kKMax + 6];
+  for (int k = 1; k <= kk + 1; k++) {
+    pres[k] = W.pres[c0 + (int64_t)(k - 1) * S];
+    prsf[k] = V.p[c0 + (int64_t)(k - 1) * S];
+  }
+  for (int k = 1; k <= kk; k++) {
+    dprs[k] = pres[k + 1] - pres[k];
+    dpi[k] = dmax(dprs[k], dpthin);
+  }
+  double* temp = V.temp + c0 + lev4(V, 1, n);
+  double* saln = V.saln + c0 + lev4(V, 1, n);
+  double* th3d = V.th3d + c0 + lev4(V, 1, n);
+  auto run_field = [&](double* f, bool ts) {
+    for (int k = 1; k <= kk; k++) sfld[k] = (k <= kp) ? f[(int64_t)(k - 1) * S] : sfld[kp];
+    auto store = [&](int k, double v) {
+      f[(int64_t)(k - 1) * S] = v;
+      if (ts) th3d[(int64_t)(k - 1) * S] = sig(temp[(int64_t)(k - 1) * S], v) - P.thbase;
+    };
+    switch (P.hybmap) {
+      case 0:
+        walk<0>(kk, sfld, pres, dprs, c1, c2, c3, prsf, dpthin, store);
+        break;
+      case 1:
+        coefs<1>(kk, sfld, dprs, dpi, lcm, dpthin, c1, c2, c3);
+        walk<1>(kk, sfld, pres, dprs, c1, c2, c3, prsf, dpthin, store);
+        break;
+      case 2:
+        coefs<2>(kk, sfld, dprs, dpi, lcm, dpthin, c1, c2, c3);
+        walk<2>(kk, sfld, pres, dprs, c1, c2, c3, prsf, dpthin, store);
+        break;
+      default:
+        coefs<3>(kk, sfld, dprs, dpi, lcm, dpthin, c1, c2, c3);
+        walk<3>(kk, sfld, pres, dprs, c1, c2, c3, prsf, dpthin, store);
+        break;
+    }
+  };
+  run_field(temp, false);
+  run_field(saln, true);
+  for (int t = 1; t <= P.ntracr; t++) run_field(V.tracer + c0 + levt(V, 1, n, t), false);
+}
+
+}  // namespace hyb

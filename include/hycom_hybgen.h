@@ -172,10 +172,11 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
 int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev);
 
 /* Kernel selection (testing and measurement; no reference counterpart).
- * 0: automatic (register-resident kernels for kdm in {22,32,41} with
- * nhybrd==kdm and no HYBGEN_DENFIX, per-lane-array kernels otherwise),
- * 1: per-lane-array kernels only, 2: register-resident kernels only
- * (HYB_EUNSUPPORTED where they do not apply).  Process-wide. */
+ * 0: automatic (streaming kernels when nhybrd==kdm and no HYBGEN_DENFIX,
+ * per-lane-array kernels otherwise), 1: per-lane-array kernels only,
+ * 2: register-resident kernels (kdm in {22,32,41} only), 3: streaming
+ * kernels only (HYB_EUNSUPPORTED where a family does not apply).
+ * Process-wide. */
 int hycom_hybgen_set_kernel_path(int path);
 
 /* Measurement hooks (no reference counterpart; the reference's only timer

@@ -157,3 +157,91 @@ extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, i
     default: return -2;
   }
 }
+
+// ---- streaming kernels (hyb_stream.h), same phase order ----
+#include "hyb_stream.h"
+
+static int64_t g_unsafe_cols = 0;
+
+static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
+  const int ii = P.ii, jj = P.jj, kk = P.kk;
+  std::vector<double> pres((size_t)V.NIJ * (kk + 1));
+  std::vector<int32_t> kpv((size_t)V.NIJ);
+  std::vector<uint64_t> lcmv((size_t)V.NIJ);
+  Scratch W{pres.data(), kpv.data(), lcmv.data()};
+  const int itr[3] = {P.itr801, P.itr802, P.itr803};
+  double* src[3] = {V.dp, V.temp, V.saln};
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          for (int k = 1; k <= kk; k++)
+            V.tracer[ij_of(V, i, j) + levt(V, k, n, itr[d])] = src[d][ij_of(V, i, j) + lev4(V, k, n)];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 1; j <= jj; j++)
+    for (int i = 1; i <= ii; i++)
+      if (V.ip[ij_of(V, i, j)]) regrid_stream(P, V, W, i, j, n);
+  int64_t unsafe = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : unsafe)
+  for (int j = 1; j <= jj; j++) {
+    double row[kKMax + 2];
+    LaneRow po{row, 1};
+    for (int i = 1; i <= ii; i++) {
+      const int64_t c0 = ij_of(V, i, j);
+      if (!V.ip[c0]) continue;
+      if (W.kp[c0] & kUnsafe) {
+        unsafe++;
+        remap_fallback(P, V, W, i, j, n);
+        continue;
+      }
+      for (int k = 1; k <= kk + 1; k++) row[k - 1] = V.p[c0 + (int64_t)(k - 1) * V.NIJ];
+      switch (P.hybmap) {
+        case 0: remap_stream<0>(P, V, W, i, j, n, po); break;
+        case 1: remap_stream<1>(P, V, W, i, j, n, po); break;
+        case 2: remap_stream<2>(P, V, W, i, j, n, po); break;
+        default: remap_stream<3>(P, V, W, i, j, n, po); break;
+      }
+    }
+  }
+  g_unsafe_cols = unsafe;
+  for (int d = 0; d < 3; d++)
+    if (itr[d] > 0) {
+      double q = d == 0 ? P.diag_q801 : P.diag_q80x;
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)])
+            for (int k = 1; k <= kk; k++) {
+              int64_t a = ij_of(V, i, j) + levt(V, k, n, itr[d]);
+              V.tracer[a] = q * (src[d][ij_of(V, i, j) + lev4(V, k, n)] - V.tracer[a]);
+            }
+    }
+  for (int pass = 0; pass <= (raflag ? 1 : 0); pass++) {
+    int nl = pass == 0 ? n : m;
+    if (pass == 1) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int j = 1; j <= jj; j++)
+        for (int i = 1; i <= ii; i++)
+          if (V.ip[ij_of(V, i, j)]) column_ra<64>(P, V, i, j, m, n, row_lcm(P));
+    }
+    for (int k = 2; k <= kk + 1; k++)
+      for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
+    for (int k = 2; k <= kk + 1; k++)
+      for (int j = 0; j <= jj + 1; j++) p_halo_icol(P, V, j, k);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 1; j <= jj; j++)
+      for (int isv = 0; isv <= 1; isv++)
+        for (int i = 1; i <= ii; i++)
+          if ((isv ? V.iv : V.iu)[ij_of(V, i, j)]) point_c<64>(P, V, i, j, nl, isv != 0);
+  }
+}
+
+extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F, int m, int n,
+                                       int raflag) {
+  KParams K = kparams_of(P);
+  View V = view_of(P, F);
+  if (K.ldenfix || K.nhybrd != K.kk) return -2;
+  run_stream(K, V, m, n, raflag);
+  return 0;
+}
+
+extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }

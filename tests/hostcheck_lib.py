@@ -1,0 +1,48 @@
+"""Loader for tests/hostcheck/libhostcheck.so: the device column code of
+the library (hycom-src_amd/csrc/hyb_*.h) compiled for the host, run in the
+kernels' phase order.  Lets the CPU suite check the kernels' own arithmetic
+against the oracle.  Test infrastructure only."""
+import ctypes
+import os
+
+import hybgen_abi as abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SO = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
+FAMILIES = {"lane-arrays": "hostcheck_hybgen", "register-resident": "hostcheck_hybgen_fast",
+            "streaming": "hostcheck_hybgen_stream"}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        h = ctypes.CDLL(SO)
+        for fn in FAMILIES.values():
+            getattr(h, fn).argtypes = [ctypes.POINTER(abi.HybParams),
+                                       ctypes.POINTER(abi.HybFields),
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        h.hostcheck_stream_unsafe_columns.restype = ctypes.c_int64
+        _lib = h
+    return _lib
+
+
+def covers(family, P):
+    ok = P.ldenfix == 0 and P.nhybrd == P.kdm
+    if family == "register-resident":
+        return ok and P.kdm in (22, 32, 41)
+    if family == "streaming":
+        return ok
+    return True
+
+
+def run(family, dom, m=1, n=2, raflag=0):
+    F = dom.fields()
+    rc = getattr(lib(), FAMILIES[family])(ctypes.byref(dom.P), ctypes.byref(F), m, n, raflag)
+    if rc != 0:
+        raise RuntimeError(f"{FAMILIES[family]} rc={rc}")
+    return dom
+
+
+def unsafe_columns():
+    return lib().hostcheck_stream_unsafe_columns()

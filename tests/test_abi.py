@@ -1,0 +1,103 @@
+"""The C ABI (include/hycom_hybgen.h) on the CPU: the library loads, exports
+every declared entry point, validates parameters like blkdat.F90, and fails
+loudly without a gfx950 device (there is no CPU fallback)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import hybgen_abi as abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hycom_hybgen.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hycom_hybgen\w*)\s*\(", txt)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for need in ("hycom_hybgen", "hycom_hybgen_device", "hycom_hybgen_check_params",
+                 "hycom_hybgen_derive_vgrid", "hycom_hybgen_strerror"):
+        assert need in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_abi_version_and_strerror():
+    lib = abi.load()
+    txt = open(HEADER).read()
+    ver = int(re.search(r"#define HYB_ABI_VERSION (\d+)", txt).group(1))
+    assert lib.hycom_hybgen_abi_version() == ver
+    lib.hycom_hybgen_strerror.restype = ctypes.c_char_p
+    for code in (0, -1, -2, -3, -4, -5, -6, -7, -8):
+        assert lib.hycom_hybgen_strerror(code)
+
+
+def _params(**over):
+    P, _ = abi.workload_params("cfg1", **over)
+    return P
+
+
+def _rc(P):
+    return abi.load().hycom_hybgen_check_params(ctypes.byref(P))
+
+
+def test_check_params_accepts_the_workloads():
+    assert _rc(_params()) == 0
+    P, _ = abi.workload_params("glbb")
+    assert _rc(P) == 0
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("hybflg", 1, -1),        # blkdat.F90:1007-1011: 17-term EOS needs hybflg 0
+    ("hybthk", 0.95, -1),     # blkdat.F90:966-974
+    ("hybthn", 0.5, -1),      # blkdat.F90:956-964
+    ("hybmap", 4, -1),
+    ("mxlmy", 1, -2),         # MY2.5 q2/q2l not built
+    ("ntracr", 65, -2),
+    ("kdm", 3, -2),
+    ("precision", 7, -1),
+])
+def test_check_params_rejects(field, value, code):
+    P = _params()
+    setattr(P, field, value)
+    assert _rc(P) == code
+
+
+def test_check_params_rejects_shared_diag_tracer():
+    P = _params(ntracr=3, trcflg=[0, 0, 0])
+    P.itracr801, P.itracr802 = 2, 2
+    assert _rc(P) == -2
+
+
+def test_check_params_rejects_lcm_carry_configuration():
+    # !isopcm with nhybrd<kdm carries lcm across the columns of a row in the
+    # reference (hybgen.F90:386-394 outside the i loop); not supported
+    P = _params(nhybrd=30, hybiso=0.0)
+    assert _rc(P) == -2
+
+
+def test_device_entry_fails_without_gpu():
+    lib = abi.load()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is visible")
+    except Exception:
+        pass
+    P = _params(idm=16, jdm=12)
+    d = abi.HostDomain(P).synth(abi.make_synth(P, abi.WORKLOADS["cfg1"], seed=1))
+    F = d.fields()
+    before = d.a["temp"].copy()
+    rc = lib.hycom_hybgen(ctypes.byref(P), ctypes.byref(F), 1, 2, 0)
+    assert rc == -5  # HYB_ENODEVICE
+    assert (d.a["temp"] == before).all()  # nothing computed on the CPU

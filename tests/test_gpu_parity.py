@@ -12,14 +12,26 @@ import oracle_lib as ol
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[1, 2], ids=["lane-arrays", "register-resident"])
+FAMILIES = {1: "lane-arrays", 2: "register-resident", 3: "streaming"}
+
+
+@pytest.fixture(params=[1, 2, 3], ids=list(FAMILIES.values()))
 def kernel_path(request, gpu):
-    """Runs the test once per kernel family (the register-resident kernels
-    cover kdm in {22,32,41}; other shapes fall back inside the library)."""
-    lib = abi.load()
-    abi.check(lib.hycom_hybgen_set_kernel_path(0 if request.param == 2 else 1))
+    """Runs the test once per kernel family.  A family that does not cover
+    the case's parameters (register-resident: kdm in {22,32,41}; both:
+    nhybrd==kdm, no HYBGEN_DENFIX) is replaced by the lane-array kernels."""
     yield request.param
-    abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    abi.check(abi.load().hycom_hybgen_set_kernel_path(0))
+
+
+def _select(family, P):
+    ok = P.ldenfix == 0 and P.nhybrd == P.kdm
+    if family == 2 and not (ok and P.kdm in (22, 32, 41)):
+        family = 1
+    if family == 3 and not ok:
+        family = 1
+    print(f"[kernels] {FAMILIES[family]}", flush=True)
+    abi.check(abi.load().hycom_hybgen_set_kernel_path(family))
 
 
 def _case(name, **over):
@@ -42,6 +54,7 @@ def _assert_same(a, b, what):
 @pytest.mark.parametrize("hybmap", [0, 1, 2, 3])
 def test_device_path_matches_oracle_cfg1(gpu, kernel_path, hybmap):
     P, w = _case("cfg1-device", hybmap=hybmap)
+    _select(kernel_path, P)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=100))
     want = _oracle_out(d0)
     dd = abi.DeviceDomain(P).upload(d0)
@@ -61,6 +74,7 @@ def test_device_path_matches_oracle_cfg1(gpu, kernel_path, hybmap):
 ])
 def test_device_path_variants(gpu, kernel_path, variant):
     P, w = _case("cfg1-variant", idm=97, jdm=61, **variant)
+    _select(kernel_path, P)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=7))
     want = _oracle_out(d0)
     dd = abi.DeviceDomain(P).upload(d0)
@@ -74,6 +88,7 @@ def test_robert_asselin_and_diag_tracers(gpu, kernel_path, mn):
     m, n = mn
     P, w = _case("raflag", idm=80, jdm=50, ntracr=4, trcflg=[2, 0, 0, 0], hybmap=2)
     P.itracr801, P.itracr802, P.itracr803 = 2, 3, 4
+    _select(kernel_path, P)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=11), m=m, n=n)
     rng = np.random.default_rng(1)
     d0.a["dpo"][...] = np.abs(d0.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d0.a["dp"].shape)))
@@ -106,6 +121,7 @@ def test_device_synth_matches_host_synth(gpu):
 def test_larger_domain_matches_oracle(gpu, kernel_path):
     P, w = _case("mid-size", idm=600, jdm=400, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0],
                  workload="glbb")
+    _select(kernel_path, P)
     S = abi.make_synth(P, w, seed=200, itdm=4500, jtdm=3298, i0=1800, j0=1200)
     dd = abi.DeviceDomain(P).synth(S)
     d0 = dd.download()
@@ -113,3 +129,25 @@ def test_larger_domain_matches_oracle(gpu, kernel_path):
     dd.hybgen(m=1, n=2)
     got = dd.download()
     _assert_same(want, got, "600x400x41")
+
+
+@pytest.mark.parametrize("kdm,hybmap,ntracr,extreme", [
+    (5, 2, 0, 0), (17, 3, 3, 0), (50, 2, 7, 1), (64, 1, 8, 0), (64, 0, 1, 1),
+    (29, 2, 9, 1), (41, 3, 4, 1), (17, 2, 2, 1),
+])
+def test_streaming_any_kdm(gpu, kdm, hybmap, ntracr, extreme):
+    """Streaming kernels on shapes the fixed-K kernels do not cover: odd and
+    maximal kdm, tracer groups beyond one walk (ntracr > 4), the in-place
+    fallback columns and the deferred clamp (extreme inputs)."""
+    dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
+    P, w = _case("streaming", idm=90, jdm=57, kdm=kdm, hybmap=hybmap, nsigma=min(8, kdm),
+                 ntracr=ntracr, trcflg=[2, 0, 0, 2, 0, 0, 0, 0, 0][:ntracr], **dx)
+    P.isopcm = 1 if kdm % 2 else 0
+    _select(3, P)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm, extreme=extreme))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    got = dd.download()
+    abi.check(abi.load().hycom_hybgen_set_kernel_path(0))
+    _assert_same(want, got, f"kdm={kdm} hybmap={hybmap} ntracr={ntracr}")

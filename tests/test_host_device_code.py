@@ -1,0 +1,65 @@
+"""The device column code of the library, compiled for the host
+(tests/hostcheck), against the oracle on seeded random parameter draws:
+the kernels' own arithmetic, bitwise, without a GPU.  The streaming
+kernels cover any kdm; the register-resident ones kdm in {22,32,41}."""
+import numpy as np
+import pytest
+
+import hybgen_abi as abi
+import hostcheck_lib as hc
+import oracle_lib as ol
+
+
+def _draw(rng, kdms):
+    kdm = int(rng.choice(kdms))
+    ntr = int(rng.integers(0, 10))
+    dx = bool(rng.integers(0, 2))
+    return dict(kdm=kdm, idm=int(rng.integers(20, 70)), jdm=int(rng.integers(10, 40)),
+                hybmap=int(rng.integers(0, 4)), nsigma=int(rng.choice([0, min(8, kdm)])),
+                hybiso=[0.0, 0.01, 3.0][int(rng.integers(0, 3))],
+                hybrlx=[1.0, 8.0][int(rng.integers(0, 2))],
+                hybthn=[1.0, 5.0][int(rng.integers(0, 2))],
+                hybthk=[0.0, 0.5, 0.9][int(rng.integers(0, 3))],
+                nreg=int(rng.integers(0, 5)), ntracr=ntr,
+                trcflg=[int(x) for x in rng.choice([0, 2], size=ntr)],
+                dx00=30.0 if dx else 9999.0, dx00f=1.1 if dx else 1.0,
+                dx00x=300.0 if dx else 9999.0)
+
+
+@pytest.mark.parametrize("family,kdms", [
+    ("streaming", [4, 5, 9, 17, 22, 29, 32, 41, 50, 63, 64]),
+    ("register-resident", [22, 32, 41]),
+    ("lane-arrays", [5, 22, 41, 64]),
+])
+@pytest.mark.parametrize("draw", range(10))
+def test_device_code_on_host_matches_oracle(family, kdms, draw):
+    rng = np.random.default_rng(9000 + 97 * draw + len(family))
+    over = _draw(rng, kdms)
+    P, w = abi.workload_params("cfg1", **over)
+    raflag = int(rng.integers(0, 2))
+    m, n = (1, 2) if rng.integers(0, 2) else (2, 1)
+    if raflag and P.ntracr >= 3:
+        P.itracr801, P.itracr802 = 1, 3
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=draw, extreme=int(rng.integers(0, 2))),
+                                m=m, n=n)
+    if raflag:
+        d.a["dpo"][...] = np.abs(d.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d.a["dp"].shape)))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, m=m, n=n, raflag=raflag, threads=4)
+    hc.run(family, b, m=m, n=n, raflag=raflag)
+    df = ol.diff_domains(a, b)
+    assert not df, f"{family} {over} raflag={raflag} m={m} n={n}: {df}"
+
+
+def test_streaming_fallback_columns_are_exercised():
+    """The in-place check of the streaming remap sends some columns of an
+    extreme domain with a maximum layer thickness (dx0k) to the copy-first
+    fallback; results stay bitwise."""
+    P, w = abi.workload_params("cfg1", idm=50, jdm=36, kdm=17, hybmap=2, ntracr=2,
+                               trcflg=[2, 0], dx00=30.0, dx00f=1.1, dx00x=300.0)
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=1, extreme=1))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, threads=4)
+    hc.run("streaming", b)
+    assert hc.unsafe_columns() > 0
+    assert not ol.diff_domains(a, b)

@@ -1,0 +1,90 @@
+"""The oracle (C restatement, oracle/hybgen_oracle.c) against the live flang
+build of the reference hybgen (oracle/_ref), on seeded random parameter
+draws beyond the golden set, and the four remappers column by column.
+Skipped where the reference build is absent (it needs /root/reference)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import hybgen_abi as abi
+import oracle_lib as ol
+
+pytestmark = pytest.mark.skipif(not (ol.ref_available() and ol.ref_available(True)),
+                                reason="oracle/_ref not built (make -C oracle/ref)")
+
+
+def _draw(rng):
+    kdm = int(rng.choice([5, 12, 22, 32, 41, 50, 64]))
+    ntr = int(rng.integers(0, 6))
+    dx = bool(rng.integers(0, 2))
+    return dict(kdm=kdm, idm=int(rng.integers(20, 60)), jdm=int(rng.integers(10, 40)),
+                hybmap=int(rng.integers(0, 4)), nsigma=int(rng.choice([0, min(8, kdm)])),
+                hybiso=[0.0, 0.01, 3.0][int(rng.integers(0, 3))],
+                hybrlx=[1.0, 3.0, 8.0][int(rng.integers(0, 3))],
+                hybthn=[1.0, 5.0][int(rng.integers(0, 2))],
+                hybthk=[0.0, 0.5, 0.9][int(rng.integers(0, 3))],
+                ldenfix=int(rng.integers(0, 2)), nreg=int(rng.integers(0, 5)),
+                ntracr=ntr, trcflg=[int(x) for x in rng.choice([0, 2], size=ntr)],
+                dx00=30.0 if dx else 9999.0, dx00f=1.1 if dx else 1.0,
+                dx00x=300.0 if dx else 9999.0)
+
+
+@pytest.mark.parametrize("draw", range(24))
+def test_oracle_matches_reference_random(draw):
+    rng = np.random.default_rng(5000 + draw)
+    over = _draw(rng)
+    P, w = abi.workload_params("cfg1", **over)
+    raflag = int(rng.integers(0, 2))
+    m, n = (1, 2) if rng.integers(0, 2) else (2, 1)
+    if raflag and P.ntracr >= 2:
+        P.itracr801, P.itracr803 = 1, 2
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=draw, extreme=int(rng.integers(0, 2))),
+                                m=m, n=n)
+    if raflag:
+        d.a["dpo"][...] = np.abs(d.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d.a["dp"].shape)))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, m=m, n=n, raflag=raflag, threads=4)
+    ol.run_ref(b, m=m, n=n, raflag=raflag)
+    df = ol.diff_domains(a, b)
+    assert not df, f"{over} raflag={raflag} m={m} n={n}: {df}"
+
+
+def _remap_pair(method, kk, si, pi, po, lcm, thin):
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.POINTER(ctypes.c_int)
+    so_a = np.zeros(kk)
+    so_b = np.zeros(kk)
+    args = (si.ctypes.data_as(D), pi.ctypes.data_as(D), po.ctypes.data_as(D),
+            lcm.ctypes.data_as(I), ctypes.c_double(thin))
+    ol.oracle().oracle_remap_column(method, kk, *args, so_a.ctypes.data_as(D))
+    ol.ref().href_remap(method, kk, *args, so_b.ctypes.data_as(D))
+    return so_a, so_b
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_remappers_match_reference_columns(method):
+    rng = np.random.default_rng(77 + method)
+    thin = 0.001 * 9.806 * 0.001 * 1000.0 * 0.001
+    for trial in range(300):
+        kk = int(rng.integers(3, 40))
+        dp = rng.exponential(1.0, kk) * (rng.random(kk) > 0.2)
+        pi = np.concatenate([[0.0], np.cumsum(dp)])
+        dq = rng.exponential(1.0, kk) * (rng.random(kk) > 0.3)
+        po = np.concatenate([[0.0], np.cumsum(dq)])
+        po *= pi[-1] / po[-1] if po[-1] > 0 else 0.0
+        si = np.cumsum(rng.standard_normal(kk))
+        lcm = (rng.random(kk) > 0.8).astype(np.int32)
+        a, b = _remap_pair(method, kk, si, pi, po, lcm, thin)
+        assert np.array_equal(a, b), f"method {method} trial {trial} kk={kk}"
+
+
+def test_eos_matches_reference_on_grid():
+    """sig() (stmt_fns.h:505-511, EOS_17T sigma-2) bitwise on a T,S grid
+    that spans the EOS_17T range and beyond (negative salinity takes the
+    max(0,s) branch of the square root)."""
+    o, r = ol.oracle(), ol.ref()
+    for t in np.linspace(-3.0, 42.0, 61):
+        for s in np.concatenate([np.linspace(-1.0, 42.0, 53), [0.0, 1e-300]]):
+            a, b = o.oracle_sig(t, s), r.href_sig(t, s)
+            assert a == b or (np.isnan(a) and np.isnan(b)), (t, s, a, b)
