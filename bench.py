@@ -92,6 +92,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-path", type=int, default=0,
                     help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
+    ap.add_argument("--remap-variant", type=int, default=None,
+                    help="streaming remap variant 0..5 (hycom_hybgen_tune knob 0)")
+    ap.add_argument("--uv-prefetch", type=int, default=None,
+                    help="streaming u/v prefetch depth 0 or 2 (knob 1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -103,6 +107,10 @@ def main():
     import torch
     lib = abi.load()
     abi.check(lib.hycom_hybgen_set_kernel_path(args.kernel_path))
+    if args.remap_variant is not None:
+        abi.check(lib.hycom_hybgen_tune(0, args.remap_variant))
+    if args.uv_prefetch is not None:
+        abi.check(lib.hycom_hybgen_tune(1, args.uv_prefetch))
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     P, w = abi.workload_params(args.workload, hybmap=args.hybmap)

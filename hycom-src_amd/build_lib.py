@@ -31,7 +31,9 @@ COMMON = [
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
           "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["hyb_kernels.hip", "hyb_capi.cpp"]
+SOURCES = ["hyb_k_remap_m2.hip", "hyb_k_remap_m3.hip", "hyb_k_remap_m1.hip",
+           "hyb_k_remap_m0.hip", "hyb_k_stream.hip", "hyb_k_lane.hip", "hyb_k_fast.hip",
+           "hyb_k_misc.hip", "hyb_capi.cpp"]
 
 
 def _run(cmd):
@@ -39,14 +41,16 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
+    """Compiles the translation units in parallel, then links."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC, "-x", "hip", *COMMON, *DEVICE, "-fopenmp", "-c",
-              os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o") for src in SOURCES]
+    cmds = [[HIPCC, "-x", "hip", *COMMON, *DEVICE, "-fopenmp", "-c",
+             os.path.join(CSRC, src), "-o", obj] for src, obj in zip(SOURCES, objs)]
+    jobs = jobs or min(len(cmds), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, *DEVICE, "-shared", "-fopenmp", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -148,16 +148,18 @@ struct Workspace {
   double* pres = nullptr;
   int32_t* kp = nullptr;
   uint64_t* lcm = nullptr;
+  int32_t* uvf = nullptr;
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
 int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays, 2 register-resident, 3 streaming
+StreamTuning g_tune;
 
 int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   for (auto& w : g_ws) {
     if (w.key == key && w.nij == nij && w.K == K) {
-      *W = Scratch{w.pres, w.kp, w.lcm};
+      *W = Scratch{w.pres, w.kp, w.lcm, w.uvf};
       return 0;
     }
   }
@@ -166,6 +168,7 @@ int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
       (void)hipFree(it->pres);
       (void)hipFree(it->kp);
       (void)hipFree(it->lcm);
+      (void)hipFree(it->uvf);
       g_ws.erase(it);
       break;
     }
@@ -177,8 +180,9 @@ int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
   HIPCHK(hipMalloc(&w.pres, sizeof(double) * nij * (K + 1)));
   HIPCHK(hipMalloc(&w.kp, sizeof(int32_t) * nij));
   HIPCHK(hipMalloc(&w.lcm, sizeof(uint64_t) * nij));
+  HIPCHK(hipMalloc(&w.uvf, sizeof(int32_t) * 2 * nij));
   g_ws.push_back(w);
-  *W = Scratch{w.pres, w.kp, w.lcm};
+  *W = Scratch{w.pres, w.kp, w.lcm, w.uvf};
   return 0;
 }
 
@@ -354,7 +358,8 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   if (use_stream) {
     Scratch W;
     if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
-    HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, (hipStream_t)stream, timer_take()));
+    HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, g_tune, (hipStream_t)stream,
+                                timer_take()));
   } else if (fast) {
     Scratch W;
     if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
@@ -369,6 +374,20 @@ int hycom_hybgen_set_kernel_path(int path) {
   if (path < 0 || path > 3) return fail(HYB_EINVAL, "kernel path %d", path);
   g_kernel_path = path;
   return 0;
+}
+
+int hycom_hybgen_tune(int knob, int value) {
+  if (knob == 0) {
+    if (value < 0 || value > 5) return fail(HYB_EINVAL, "remap variant %d outside 0..5", value);
+    g_tune.remap_variant = value;
+    return 0;
+  }
+  if (knob == 1) {
+    if (value != 0 && value != 2) return fail(HYB_EINVAL, "u/v prefetch %d: 0 or 2", value);
+    g_tune.uv_prefetch = value;
+    return 0;
+  }
+  return fail(HYB_EINVAL, "tuning knob %d", knob);
 }
 
 int hycom_hybgen_timing(int enable) {

@@ -26,6 +26,7 @@ struct Scratch {
   double* pres;       // (NIJ, K+1) old interfaces after overturn/unmix
   int32_t* kp;        // deepest inflated layer after overturn
   uint64_t* lcm;      // PCM-layer mask after the dx0k pass
+  int32_t* uvf;       // (2, NIJ) u/v points left to the copy-first kernel
 };
 
 // Row of per-lane storage in LDS (device) or a local array (host).

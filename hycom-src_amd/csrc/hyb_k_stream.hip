@@ -1,0 +1,97 @@
+// hyb_k_stream.hip -- the streaming family (hyb_stream.h): regrid, u/v
+// kernels and the launcher of the whole call, any kdm <= 64.
+#include "hyb_launch.h"
+#include "hyb_stream.h"
+
+namespace hyb {
+
+__global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  if (V.ip[ij_of(V, i, j)] == 0) return;
+  regrid_stream(P, V, W, i, j, n);
+}
+
+template <int METHOD, int PD>
+__global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, int nl) {
+  extern __shared__ double lds[];
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  bool isv = blockIdx.z != 0;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  const int32_t* msk = isv ? V.iv : V.iu;
+  if (msk[c0] == 0) return;
+  LaneRow dcol{lds + threadIdx.x, kWave};
+  const bool ok = uv_stream<METHOD, PD>(P, V, i, j, nl, isv, dcol);
+  W.uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
+}
+
+static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, int nl,
+                             int pd, hipStream_t s) {
+  dim3 w1(kWave);
+  dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
+  const size_t lds = sizeof(double) * P.kk * kWave;
+  const int m = uv_method(P);
+  if (pd > 0) {
+    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 2>), gw3, w1, lds, s, P, V, W, nl);
+    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 2>), gw3, w1, lds, s, P, V, W, nl);
+    else hipLaunchKernelGGL((k_uv_s<3, 2>), gw3, w1, lds, s, P, V, W, nl);
+  } else {
+    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 0>), gw3, w1, lds, s, P, V, W, nl);
+    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 0>), gw3, w1, lds, s, P, V, W, nl);
+    else hipLaunchKernelGGL((k_uv_s<3, 0>), gw3, w1, lds, s, P, V, W, nl);
+  }
+  launch_uv_fallback(P, V, W, nl, s);
+}
+
+bool stream_path_ok(const KParams& P) { return !P.ldenfix && P.nhybrd == P.kk; }
+
+hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
+                                int n, int raflag, const StreamTuning& tune, hipStream_t s,
+                                PhaseTimer* T) {
+  if (T)
+    for (int q = 0; q < kPhases; q++) T->used[q] = false;
+  dim3 blk(kBlock);
+  dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
+  launch_diag_pre(P, V, n, s);
+  {
+    PhaseMark pm(T, s, 0);
+    hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
+    switch (P.hybmap) {
+      case 0: launch_remap_stream_m0(P, V, W, n, tune.remap_variant, s); break;
+      case 1: launch_remap_stream_m1(P, V, W, n, tune.remap_variant, s); break;
+      case 2: launch_remap_stream_m2(P, V, W, n, tune.remap_variant, s); break;
+      default: launch_remap_stream_m3(P, V, W, n, tune.remap_variant, s); break;
+    }
+    launch_remap_fallback(P, V, W, n, s);
+  }
+  launch_diag_post(P, V, n, s);
+  hipError_t e;
+  {
+    PhaseMark pm(T, s, 1);
+    e = launch_halo(P, V, s);
+  }
+  if (e != hipSuccess) return e;
+  {
+    PhaseMark pm(T, s, 2);
+    launch_uv_stream(P, V, W, n, tune.uv_prefetch, s);
+  }
+  if (raflag) {
+    {
+      PhaseMark pm(T, s, 3);
+      launch_ra_lane(P, V, m, n, s);
+    }
+    {
+      PhaseMark pm(T, s, 4);
+      e = launch_halo(P, V, s);
+    }
+    if (e != hipSuccess) return e;
+    PhaseMark pm(T, s, 5);
+    launch_uv_stream(P, V, W, m, tune.uv_prefetch, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hyb

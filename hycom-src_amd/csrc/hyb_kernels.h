@@ -32,9 +32,18 @@ hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W,
 // True when the streaming kernels cover this parameter pack (any kdm,
 // nhybrd==kdm, no HYBGEN_DENFIX).
 bool stream_path_ok(const KParams& P);
-// Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast.
+// Kernel variants of the streaming family (measurement and tuning):
+// remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
+// uv_prefetch the prefetch depth of the u/v walk (0 or 2).
+struct StreamTuning {
+  int remap_variant = 0;
+  int uv_prefetch = 0;
+};
+// Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
+// plus uvf (2*NIJ int32).
 hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
-                                int n, int raflag, hipStream_t s, PhaseTimer* T = nullptr);
+                                int n, int raflag, const StreamTuning& tune, hipStream_t s,
+                                PhaseTimer* T = nullptr);
 
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,

@@ -605,19 +605,132 @@ HYB_HD void regrid_stream(const KParams& P, const View& V, const Scratch& W, int
 }
 
 // ---------------------------------------------------------------------
-// remap of up to kNFMax fields of one column in one walk over the old
-// layers.  po(k) = prsf(k), the new interfaces (k = 1..kk+1).
+// Sources of the old interfaces pi(x) for walk_core: pi(1), then pi(x+1)
+// from pi(x) (the walk asks for them in increasing x, and scans ahead).
 // ---------------------------------------------------------------------
-template <int METHOD, class PO>
-HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, double zx,
-                        const PO& po, double* const* fp, int nf, double* th3d, int kp,
-                        uint64_t lcm) {
-  constexpr int NF = kNFMax;
+// raw(x) is the memory word that gives pi(x+1) from pi(x) through comb(),
+// so a walk can issue it early.
+struct PiArray {  // p-columns: the overturned interfaces left by regrid_stream
+  const double* pres;
+  int64_t S;
+  HYB_HD double first() const { return pres[0]; }
+  HYB_HD double raw(int x) const { return pres[(int64_t)x * S]; }
+  HYB_HD double comb(double, double r) const { return r; }
+  HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
+};
+struct PiSum {  // u/v points: running sums of the old dpu (hybgen.F90:171-195)
+  const double* dpu;
+  int64_t S;
+  HYB_HD double first() const { return 0.0; }
+  HYB_HD double raw(int x) const { return dpu[(int64_t)(x - 1) * S]; }
+  HYB_HD double comb(double pix, double r) const { return pix + r; }
+  HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
+};
+// New interfaces prsf(k): p-columns keep them in the lane's LDS column
+// (LaneRow); u/v points keep the new thicknesses there and sum them as
+// the walk asks for prsf(1), prsf(2), ... (hybgen.F90:1806-1810).
+struct PoSum {
+  LaneRow d;
+  double prv = 0.0, cur = 0.0;  // prsf(k-1), prsf(k) of the last request k
+  int k = 1;
+  HYB_HD double operator()(int kq) {
+    if (kq == 1) {
+      prv = 0.0;
+      cur = 0.0;
+      k = 1;
+      return cur;
+    }
+    while (k < kq) {
+      prv = cur;
+      cur = cur + d(k);
+      k++;
+    }
+    return cur;
+  }
+};
+
+// Output sinks.  emit(kc, v, certain, thick): v[f] is output kc of field
+// f before the clamp; certain means it lies inside the range of the input
+// levels read so far, so the clamp against the whole column cannot change
+// it.  fixup() clamps the others once the column range is known.
+template <int NF>
+struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
+  double* const* fp;
+  int nf;
+  double* th3d;
+  double thbase;
+  int64_t S;
+  uint64_t fix = 0;
+  HYB_HD void emit(int kc, const double* v, bool certain, bool) {
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+      if (f < nf) fp[f][(int64_t)(kc - 1) * S] = v[f];
+    if (!certain) {
+      fix |= 1ull << (kc - 1);
+    } else if (th3d) {
+      th3d[(int64_t)(kc - 1) * S] = sig(v[0], v[1]) - thbase;
+    }
+  }
+  HYB_HD void fixup(const double* smin, const double* smax) {
+    while (fix) {
+      const int k = __builtin_ctzll(fix) + 1;
+      fix &= fix - 1;
+      double v[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        v[f] = 0.0;
+        if (f < nf) {
+          double x = fp[f][(int64_t)(k - 1) * S];
+          x = dmax(smin[f], x);
+          x = dmin(smax[f], x);
+          fp[f][(int64_t)(k - 1) * S] = x;
+          v[f] = x;
+        }
+      }
+      if (th3d) th3d[(int64_t)(k - 1) * S] = sig(v[0], v[1]) - thbase;
+    }
+  }
+};
+struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
+  double* uu;
+  int64_t S;
+  double ub;    // ubavg or vbavg at time level nl
+  double ptop;  // prsf(kk+1)
+  const PoSum* po;
+  uint64_t fix = 0;
+  HYB_HD void emit(int kc, const double* v, bool certain, bool thick) {
+    if (thick || po->prv <= ptop - kOnemm) {
+      uu[(int64_t)(kc - 1) * S] = v[0];
+      if (!certain) fix |= 1ull << (kc - 1);
+    } else {
+      uu[(int64_t)(kc - 1) * S] = -ub;
+    }
+  }
+  HYB_HD void fixup(const double* smin, const double* smax) {
+    while (fix) {
+      const int k = __builtin_ctzll(fix) + 1;
+      fix &= fix - 1;
+      double x = uu[(int64_t)(k - 1) * S];
+      x = dmax(smin[0], x);
+      x = dmin(smax[0], x);
+      uu[(int64_t)(k - 1) * S] = x;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------
+// remap of up to NF fields of one column in one walk over the old layers
+// (hybgen_{pcm,plm,ppm,weno}_{coefs,remap}).  fp[f] points at level 1 of
+// field f (stride S); levels > kp take level kp's value (the massless
+// fill); zx = pi(kk+1).  PD levels of every field (and of the old
+// interfaces) are loaded ahead of the window, to cover memory latency.
+// ---------------------------------------------------------------------
+template <int METHOD, int NF, int PD, class PI, class PO, class SINK>
+HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
+                      int nf, int kp, uint64_t lcm, int64_t S) {
+  constexpr int PQ = PD > 0 ? PD : 1;
   const double thin = kDpthin;
   const double dsmll = 1.0e-8;
-  auto pi_at = [&](int x) -> double {
-    return (x >= 1 && x <= K + 1) ? pres[(int64_t)(x - 1) * S] : 0.0;
-  };
   auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
 
   // per-field state
@@ -627,6 +740,9 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
   double asc[NF], alc[NF];                     // PPM: as(l), raw al(l)
   double wc1[NF], wc2[NF], wz1[NF], wz2[NF];   // WENO first sweep of layer l
   double zw3l[NF];                             // WENO interface value zw3(l)
+  double sq[NF][PQ];                           // s(l+3..l+2+PD), loaded ahead
+  double prq[PQ];                              // raw(l+3..l+2+PD) of pi
+  int xl = 4;                                  // next level to load
   // s(x) with the massless fill (972-996) and the running min/max
   // (e.g. 2727-2731: si_min/si_max over k = 1..kk in order)
   auto load = [&](int f, int x) -> double {
@@ -658,6 +774,8 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
       if (K >= 2) s2[f] = load(f, 2);
       if (K >= 3) s3[f] = load(f, 3);
       sfirst[f] = s1[f];
+#pragma unroll
+      for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? load(f, 4 + q) : 0.0;
       if (METHOD == 2) alc[f] = s1[f];  // al(1) = s(1)
       if (METHOD == 3) {                // first sweep of layer 1
         wc1[f] = s1[f];
@@ -666,7 +784,18 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
     }
   }
   // interfaces pi(l-1..l+3) at l = 1
-  double pv0 = 0.0, pv1 = pi_at(1), pv2 = pi_at(2), pv3 = pi_at(3), pv4 = pi_at(4);
+  double pv0 = 0.0, pv1 = pi.first(), pv2 = 0.0, pv3 = 0.0, pv4 = 0.0;
+  if (K + 1 >= 2) pv2 = pi.nextfrom(1, pv1);
+  if (K + 1 >= 3) pv3 = pi.nextfrom(2, pv2);
+  if (K + 1 >= 4) pv4 = pi.nextfrom(3, pv3);
+#pragma unroll
+  for (int q = 0; q < PD; q++) prq[q] = (4 + q <= K) ? pi.raw(4 + q) : 0.0;
+  xl = 4 + PD;
+  // layers thicker than thin, known up to l+2 (the u/v store rule)
+  uint64_t thick = 0;
+  if (dmax(pv2 - pv1, thin) > thin) thick |= 1ull;
+  if (K >= 2 && dmax(pv3 - pv2, thin) > thin) thick |= 2ull;
+  if (K >= 3 && dmax(pv4 - pv3, thin) > thin) thick |= 4ull;
   // carried metrics: PPM qdpjp(l); WENO 1/(dp(l)+dp(l+1))
   double qdpjp_l = 0.0, wq_l = 0.0;
   if (K >= 2) {
@@ -676,31 +805,33 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
   }
 
   // walk state (2729-2735)
-  const double zb0 = dmax(po(1), pi_at(1));
+  const double zb0 = dmax(po(1), pv1);
   int lt_c = 1;
-  while (pi_at(lt_c + 1) < zb0 && lt_c < K) lt_c++;
+  {
+    double pc = pv1;
+    while (lt_c < K) {
+      const double pn = pi.nextfrom(lt_c, pc);
+      if (!(pn < zb0)) break;
+      lt_c++;
+      pc = pn;
+    }
+  }
   double zt_c = zb0;
   int kc = 1;
   bool accon = false, acc_thin = false;
   int lb_act = 0;
   double zt_act = 0.0, zb_act = 0.0;
-  uint64_t fix = 0;  // outputs stored unclamped
 
-  // store output kc; th3d when T and S are in this group
+  // output kc complete: v[f] before the clamp
   auto emit = [&](const double* v) {
     bool certain = true;
 #pragma unroll
     for (int f = 0; f < NF; f++)
       if (f < nf) {
-        fp[f][(int64_t)(kc - 1) * S] = v[f];
         prev[f] = v[f];
         if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
       }
-    if (!certain) {
-      fix |= 1ull << (kc - 1);
-    } else if (th3d) {
-      th3d[(int64_t)(kc - 1) * S] = sig(v[0], v[1]) - P.thbase;
-    }
+    sink.emit(kc, v, certain, (thick >> (kc - 1)) & 1ull);
     kc = kc + 1;
   };
 
@@ -893,6 +1024,17 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
     }
 
     const double pil = pv1, pil1 = pv2, dprsl = dprs_0;
+    // Every output still to come ends in this layer when it reaches the
+    // bottom, and the walk stops here: fold the levels not read yet into
+    // the column range now, before any of them is overwritten.
+    if (l < K && pil1 >= zx) {
+      for (int x = xl; x <= K; x++) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+          if (f < nf) (void)load(f, x);
+      }
+      xl = K + 1;
+    }
     // ------------------------------------------ output in progress
     if (accon) {
       if (lb_act == l) {
@@ -1009,7 +1151,13 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
             lb = l + 2;
           } else {
             lb = l + 3;
-            while (lb < K && pi_at(lb + 1) < zb) lb++;
+            double pc = pv4;  // pi(l+3)
+            while (lb < K) {
+              const double pn = pi.nextfrom(lb, pc);
+              if (!(pn < zb)) break;
+              lb++;
+              pc = pn;
+            }
           }
           lb_act = lb;
           zt_act = zt;
@@ -1063,43 +1211,42 @@ HYB_HD void walk_stream(const KParams& P, const double* pres, int64_t S, int K, 
     }
     if (kc > K) break;
     // ------------------------------------------ slide to layer l+1
+    const double top = pv4;
     pv0 = pv1;
     pv1 = pv2;
     pv2 = pv3;
-    pv3 = pv4;
-    pv4 = pi_at(l + 4);
+    pv3 = top;
+    if (PD == 0) {
+      pv4 = (l + 4 <= K + 1) ? pi.nextfrom(l + 3, top) : 0.0;
+    } else {
+      pv4 = (l + 4 <= K + 1) ? pi.comb(top, prq[0]) : 0.0;
+#pragma unroll
+      for (int q = 0; q < PD - 1; q++) prq[q] = prq[q + 1];
+      prq[PQ - 1] = (l + 3 + PD <= K) ? pi.raw(l + 3 + PD) : 0.0;
+    }
+    if (l + 3 <= K && dmax(pv4 - pv3, thin) > thin) thick |= 1ull << (l + 2);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       s0[f] = s1[f];
       s1[f] = s2[f];
       s2[f] = s3[f];
-      s3[f] = 0.0;
-      if (f < nf && l + 3 <= K) s3[f] = load(f, l + 3);
-    }
-  }
-  // clamp of the outputs stored unclamped (running range -> column range)
-  while (fix) {
-    const int k = __builtin_ctzll(fix) + 1;
-    fix &= fix - 1;
-    double v[NF];
+      if (PD == 0) {
+        s3[f] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
+      } else {
+        s3[f] = sq[f][0];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      v[f] = 0.0;
-      if (f < nf) {
-        double x = fp[f][(int64_t)(k - 1) * S];
-        x = dmax(smin[f], x);
-        x = dmin(smax[f], x);
-        fp[f][(int64_t)(k - 1) * S] = x;
-        v[f] = x;
+        for (int q = 0; q < PD - 1; q++) sq[f][q] = sq[f][q + 1];
+        sq[f][PQ - 1] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
       }
     }
-    if (th3d) th3d[(int64_t)(k - 1) * S] = sig(v[0], v[1]) - P.thbase;
+    xl++;
   }
+  sink.fixup(smin, smax);
 }
 
-template <int METHOD, class PO>
+template <int METHOD, int NF, int PD, class PO>
 HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
-                         const PO& po) {
+                         PO& po) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
   const int64_t S = V.NIJ;
@@ -1110,11 +1257,11 @@ HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int 
   const double* pres = W.pres + c0;
   const double zx = pres[(int64_t)K * S];
   const int nfields = 2 + P.ntracr;
-  for (int g0 = 0; g0 < nfields; g0 += kNFMax) {
-    const int nf = (nfields - g0 < kNFMax) ? nfields - g0 : kNFMax;
-    double* fp[kNFMax];
+  for (int g0 = 0; g0 < nfields; g0 += NF) {
+    const int nf = (nfields - g0 < NF) ? nfields - g0 : NF;
+    double* fp[NF];
 #pragma unroll
-    for (int f = 0; f < kNFMax; f++) {
+    for (int f = 0; f < NF; f++) {
       const int g = g0 + f;
       if (f >= nf) fp[f] = nullptr;
       else if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
@@ -1122,8 +1269,77 @@ HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int 
       else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
     }
     double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
-    walk_stream<METHOD>(P, pres, S, K, zx, po, fp, nf, th, kp, lcm);
+    PiArray pi{pres, S};
+    FieldSink<NF> sink{fp, nf, th, P.thbase, S};
+    walk_core<METHOD, NF, PD>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
   }
+}
+
+// ---------------------------------------------------------------------
+// One u- or v-point: the new thickness (dpudpv.F90:92-103), the old
+// interfaces as running sums of the old dpu (hybgen.F90:171-195), the new
+// ones as running sums of the new dpu (1806-1810), and the remap of u or v
+// with the thin-layer rule (hybgenbj, 1758-1907).  dcol is the lane's LDS
+// column for the new thicknesses.  Where an output of the in-place walk
+// would overwrite a level not yet read (the test of regrid_stream), it
+// writes pu only and returns false; point_c then redoes the point.
+// ---------------------------------------------------------------------
+template <int METHOD, int PD>
+HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bool isv,
+                      const LaneRow& dcol) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int64_t cn = isv ? ij_of(V, i, j - 1) : ij_of(V, i - 1, j);
+  double* dpu = (isv ? V.dpv : V.dpu) + c0 + lev4(V, 1, nl);
+  double* uu = (isv ? V.v : V.u) + c0 + lev4(V, 1, nl);
+  double* pu = (isv ? V.pv : V.pu) + c0;
+  const double depthu = (isv ? V.depthv : V.depthu)[c0];
+  const double ub = (isv ? V.vbavg : V.ubavg)[c0 + lev(V, nl)];
+  const double* p0 = V.p + c0;
+  const double* p1 = V.p + cn;
+  double b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
+  double pr = 0.0;  // prsf(k+1)
+  pu[0] = 0.0;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;  // pres(k-2), pres(k-1), pres(k)
+  bool have = false;
+  double cand = 0.0;
+#pragma unroll 2
+  for (int k = 1; k <= K; k++) {
+    const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
+    const double d = dmax(0.0, a - b);
+    b = a;
+    dcol(k) = d;
+    pr = pr + d;
+    pu[(int64_t)k * S] = pr;
+    // output k ends in old layer <= k-3 only if pres(k-2) >= prsf(k+1)
+    if (k >= 4 && q0 >= pr && (!have || pr < cand)) {
+      cand = pr;
+      have = true;
+    }
+    const double q3 = q2 + dpu[(int64_t)(k - 1) * S];
+    q0 = q1;
+    q1 = q2;
+    q2 = q3;
+  }
+  const double zx = q2;
+  if (have && cand < zx) return false;
+  PiSum pi{dpu, S};
+  PoSum po{dcol};
+  UvSink sink{uu, S, ub, pr, &po};
+  double* fp[1] = {uu};
+  walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+#pragma unroll 2
+  for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
+  return true;
+}
+
+// remapper of hybgenbj (hybgen.F90:1819-1823): PCM, PLM when hybiso > 2,
+// otherwise WENO
+HYB_HD int uv_method(const KParams& P) {
+  if (P.hybmap == 0) return 0;
+  if (P.hybmap == 1 && P.hybiso > 2.0) return 1;
+  return 3;
 }
 
 // ---------------------------------------------------------------------

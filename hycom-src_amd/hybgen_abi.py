@@ -75,6 +75,7 @@ SIGNATURES = {
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "hycom_hybgen_set_kernel_path": (ctypes.c_int, [ctypes.c_int]),
+    "hycom_hybgen_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_timing": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_timing_read": (ctypes.c_int, [_P(ctypes.c_double), _P(ctypes.c_int32),
                                                 ctypes.c_int]),

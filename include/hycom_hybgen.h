@@ -179,6 +179,13 @@ int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev);
  * Process-wide. */
 int hycom_hybgen_set_kernel_path(int path);
 
+/* Variants of the streaming kernels (measurement and tuning; no reference
+ * counterpart; results are bitwise the same for every value).  knob 0:
+ * p-column remap, value 0..5 = (fields per walk, levels loaded ahead)
+ * (6,0) (6,2) (3,0) (3,2) (2,2) (2,4); knob 1: levels loaded ahead by the
+ * u/v walk, 0 or 2.  Process-wide. */
+int hycom_hybgen_tune(int knob, int value);
+
 /* Measurement hooks (no reference counterpart; the reference's only timer
  * is xctmr 49 around the whole call, mod_hycom.F90:2648-2653).  While
  * enabled, every hycom_hybgen_device() call records HIP events around its

@@ -162,6 +162,7 @@ extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, i
 #include "hyb_stream.h"
 
 static int64_t g_unsafe_cols = 0;
+static int64_t g_uv_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
   const int ii = P.ii, jj = P.jj, kk = P.kk;
@@ -196,10 +197,10 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
       }
       for (int k = 1; k <= kk + 1; k++) row[k - 1] = V.p[c0 + (int64_t)(k - 1) * V.NIJ];
       switch (P.hybmap) {
-        case 0: remap_stream<0>(P, V, W, i, j, n, po); break;
-        case 1: remap_stream<1>(P, V, W, i, j, n, po); break;
-        case 2: remap_stream<2>(P, V, W, i, j, n, po); break;
-        default: remap_stream<3>(P, V, W, i, j, n, po); break;
+        case 0: remap_stream<0, kNFMax, 0>(P, V, W, i, j, n, po); break;
+        case 1: remap_stream<1, 3, 2>(P, V, W, i, j, n, po); break;
+        case 2: remap_stream<2, kNFMax, 2>(P, V, W, i, j, n, po); break;
+        default: remap_stream<3, 2, 4>(P, V, W, i, j, n, po); break;
       }
     }
   }
@@ -227,11 +228,27 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
       for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
     for (int k = 2; k <= kk + 1; k++)
       for (int j = 0; j <= jj + 1; j++) p_halo_icol(P, V, j, k);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int j = 1; j <= jj; j++)
+    int64_t uv_fb = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : uv_fb)
+    for (int j = 1; j <= jj; j++) {
+      double col[kKMax + 2];
+      LaneRow dcol{col, 1};
       for (int isv = 0; isv <= 1; isv++)
-        for (int i = 1; i <= ii; i++)
-          if ((isv ? V.iv : V.iu)[ij_of(V, i, j)]) point_c<64>(P, V, i, j, nl, isv != 0);
+        for (int i = 1; i <= ii; i++) {
+          if (!(isv ? V.iv : V.iu)[ij_of(V, i, j)]) continue;
+          bool ok;
+          switch (uv_method(P)) {
+            case 0: ok = uv_stream<0, 2>(P, V, i, j, nl, isv != 0, dcol); break;
+            case 1: ok = uv_stream<1, 0>(P, V, i, j, nl, isv != 0, dcol); break;
+            default: ok = uv_stream<3, 2>(P, V, i, j, nl, isv != 0, dcol); break;
+          }
+          if (!ok) {
+            uv_fb++;
+            point_c<64>(P, V, i, j, nl, isv != 0);
+          }
+        }
+    }
+    g_uv_fallback += uv_fb;
   }
 }
 
@@ -240,8 +257,11 @@ extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F,
   KParams K = kparams_of(P);
   View V = view_of(P, F);
   if (K.ldenfix || K.nhybrd != K.kk) return -2;
+  g_uv_fallback = 0;
   run_stream(K, V, m, n, raflag);
   return 0;
 }
 
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }
+
+extern "C" int64_t hostcheck_stream_uv_fallback_points(void) { return g_uv_fallback; }

@@ -23,6 +23,7 @@ def lib():
                                        ctypes.POINTER(abi.HybFields),
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
         h.hostcheck_stream_unsafe_columns.restype = ctypes.c_int64
+        h.hostcheck_stream_uv_fallback_points.restype = ctypes.c_int64
         _lib = h
     return _lib
 
@@ -46,3 +47,7 @@ def run(family, dom, m=1, n=2, raflag=0):
 
 def unsafe_columns():
     return lib().hostcheck_stream_unsafe_columns()
+
+
+def uv_fallback_points():
+    return lib().hostcheck_stream_uv_fallback_points()

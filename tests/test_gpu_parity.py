@@ -141,8 +141,8 @@ def test_streaming_any_kdm(gpu, kdm, hybmap, ntracr, extreme):
     fallback columns and the deferred clamp (extreme inputs)."""
     dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
     P, w = _case("streaming", idm=90, jdm=57, kdm=kdm, hybmap=hybmap, nsigma=min(8, kdm),
-                 ntracr=ntracr, trcflg=[2, 0, 0, 2, 0, 0, 0, 0, 0][:ntracr], **dx)
-    P.isopcm = 1 if kdm % 2 else 0
+                 ntracr=ntracr, trcflg=[2, 0, 0, 2, 0, 0, 0, 0, 0][:ntracr],
+                 hybiso=0.01 if kdm % 2 else 0.0, **dx)
     _select(3, P)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm, extreme=extreme))
     want = _oracle_out(d0)

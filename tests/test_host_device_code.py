@@ -63,3 +63,19 @@ def test_streaming_fallback_columns_are_exercised():
     hc.run("streaming", b)
     assert hc.unsafe_columns() > 0
     assert not ol.diff_domains(a, b)
+
+
+def test_streaming_uv_fallback_points_are_exercised():
+    """u/v points whose in-place walk is not safe go to point_c; a domain
+    with massless upper layers at u/v points (old dpu = 0) has them."""
+    found = 0
+    for seed in range(6):
+        P, w = abi.workload_params("cfg1", idm=60, jdm=40, hybmap=int(seed % 4), hybrlx=1.0,
+                                   hybthn=5.0, nsigma=0, nreg=3)
+        d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=1000 + seed))
+        a, b = d.copy(), d.copy()
+        ol.run_oracle(a, threads=4)
+        hc.run("streaming", b)
+        found += hc.uv_fallback_points()
+        assert not ol.diff_domains(a, b)
+    assert found > 0
