@@ -176,23 +176,12 @@
       enddo
       end subroutine set_params
 
-! Runs the reference hybgen(m,n,raflag) on the caller's arrays, in place.
-! secs returns the wall time of the hybgen call alone (no copies).
-      integer(c_int) function href_hybgen(prm, fld, m, n, raflag, secs) &
-          bind(C, name='href_hybgen')
+! Loads the module state from the caller's arrays (ensure_alloc and
+! set_params first); store_state copies the fields hybgen writes back.
+      subroutine load_state(prm, fld)
       type(hyb_params_t), intent(in) :: prm
       type(hyb_fields_t), intent(in) :: fld
-      integer(c_int), value :: m, n, raflag
-      real(c_double), intent(out) :: secs
       integer :: n2, n3, n4, n5, n3p, ntr
-      integer(8) :: t0, t1, rate
-      external hybgen
-      href_hybgen = -1
-      secs = 0.0
-      if (prm%nbdy .ne. nbdy) return
-      if (prm%hybflg .ne. 0 .or. prm%mxlmy .ne. 0) return
-      call ensure_alloc(prm)
-      call set_params(prm)
       ntr = max(prm%ntracr, 1)
       n2  = (idm+2*nbdy)*(jdm+2*nbdy)
       n3  = n2*kdm
@@ -222,10 +211,18 @@
       call cin(depthv, fld%depthv, n2)
       call cin(ubavg,  fld%ubavg,  n2*3)
       call cin(vbavg,  fld%vbavg,  n2*3)
-      call system_clock(t0, rate)
-      call hybgen(int(m), int(n), raflag .ne. 0)
-      call system_clock(t1)
-      secs = real(t1-t0, 8)/real(rate, 8)
+      end subroutine load_state
+
+      subroutine store_state(prm, fld)
+      type(hyb_params_t), intent(in) :: prm
+      type(hyb_fields_t), intent(in) :: fld
+      integer :: n2, n3, n4, n5, n3p, ntr
+      ntr = max(prm%ntracr, 1)
+      n2  = (idm+2*nbdy)*(jdm+2*nbdy)
+      n3  = n2*kdm
+      n3p = n2*(kdm+1)
+      n4  = n3*2
+      n5  = n4*ntr
       call cout(dp,   fld%dp,   n4)
       call cout(dpo,  fld%dpo,  n4)
       call cout(dpu,  fld%dpu,  n4)
@@ -239,83 +236,32 @@
       call cout(p,  fld%p,  n3p)
       call cout(pu, fld%pu, n3p)
       call cout(pv, fld%pv, n3p)
+      end subroutine store_state
+
+! Runs hybgen(m,n,raflag) -- the reference's, or whatever subroutine hybgen
+! the library is linked with -- on the caller's arrays, in place.  secs
+! returns the wall time of the hybgen call alone (no copies).
+      integer(c_int) function href_hybgen(prm, fld, m, n, raflag, secs) &
+          bind(C, name='href_hybgen')
+      type(hyb_params_t), intent(in) :: prm
+      type(hyb_fields_t), intent(in) :: fld
+      integer(c_int), value :: m, n, raflag
+      real(c_double), intent(out) :: secs
+      integer(8) :: t0, t1, rate
+      external hybgen
+      href_hybgen = -1
+      secs = 0.0
+      if (prm%nbdy .ne. nbdy) return
+      if (prm%hybflg .ne. 0 .or. prm%mxlmy .ne. 0) return
+      call ensure_alloc(prm)
+      call set_params(prm)
+      call load_state(prm, fld)
+      call system_clock(t0, rate)
+      call hybgen(int(m), int(n), raflag .ne. 0)
+      call system_clock(t1)
+      secs = real(t1-t0, 8)/real(rate, 8)
+      call store_state(prm, fld)
       href_hybgen = 0
       end function href_hybgen
-
-! sig(t,s) of the reference equation of state (stmt_fns.h:505-511).
-      real(c_double) function href_sig(xt, xs) bind(C, name='href_sig')
-      real(c_double), value :: xt, xs
-#include "stmt_fns.h"
-      href_sig = sig(xt, xs)
-      end function href_sig
-
-! The reference's physical constants (mod_cb_arrays.F90:843-857) and the
-! folded EOS coefficients (stmt_fns.h:251-262), for bitwise pinning.
-      subroutine href_consts(c) bind(C, name='href_consts')
-      real(c_double), intent(out) :: c(16)
-#include "stmt_fns.h"
-      c(1) = onem
-      c(2) = tenm
-      c(3) = onemm
-      c(4) = epsil
-      c(5) = qonem
-      c(6) = svref
-      c(7) = c101
-      c(8) = c103
-      c(9) = c105
-      c(10) = c108
-      c(11) = c109
-      c(12) = c111
-      c(13) = rpdb
-      c(14) = 0.001*onemm
-      c(15) = 0.0
-      c(16) = 0.0
-      end subroutine href_consts
-
-! One column through the reference remapper pair, as hybgenaj calls them
-! (hybgen.F90:1530-1548): method 0 PCM, 1 PLM, 2 PPM, 3 WENO.
-      subroutine href_remap(method, kin, si, pin, po, lcm_i, thin, so) &
-          bind(C, name='href_remap')
-      integer(c_int), value :: method, kin
-      real(c_double), intent(in) :: si(kin), pin(kin+1), po(kin+1)
-      integer(c_int), intent(in) :: lcm_i(kin)
-      real(c_double), value :: thin
-      real(c_double), intent(out) :: so(kin)
-      real :: dprs(kin), dpi(kin), c1d(kin,1,3), s1(kin,1), f1(kin,1)
-      real :: pi1(kin+1), po1(kin+1), th
-      logical :: lcm(kin)
-      integer :: k
-      external hybgen_pcm_remap, hybgen_plm_coefs, hybgen_plm_remap, &
-               hybgen_ppm_coefs, hybgen_ppm_remap, &
-               hybgen_weno_coefs, hybgen_weno_remap
-      th = thin
-      do k= 1,kin
-        s1(k,1) = si(k)
-        lcm(k) = lcm_i(k) .ne. 0
-      enddo
-      do k= 1,kin+1
-        pi1(k) = pin(k)
-        po1(k) = po(k)
-      enddo
-      do k= 1,kin
-        dprs(k) = pi1(k+1)-pi1(k)
-        dpi(k)  = max(dprs(k),th)
-      enddo
-      if     (method.eq.0) then
-        call hybgen_pcm_remap(s1,pi1,dprs,f1,po1,kin,kin,1,th)
-      elseif (method.eq.1) then
-        call hybgen_plm_coefs(s1,dprs,lcm,c1d,kin,1,th)
-        call hybgen_plm_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
-      elseif (method.eq.2) then
-        call hybgen_ppm_coefs(s1,dpi,lcm,c1d,kin,1,th)
-        call hybgen_ppm_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
-      else
-        call hybgen_weno_coefs(s1,dpi,lcm,c1d,kin,1,th)
-        call hybgen_weno_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
-      endif
-      do k= 1,kin
-        so(k) = f1(k,1)
-      enddo
-      end subroutine href_remap
 
       end module hybgen_ref_driver

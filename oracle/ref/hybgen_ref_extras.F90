@@ -1,0 +1,90 @@
+! ORACLE HARNESS -- test infrastructure, not product code.
+!
+! C-callable pieces of the reference besides hybgen itself: the equation
+! of state, the constants and one column through a remapper pair.  Kept
+! apart from hybgen_ref_driver.F90 so that the shim check library, which
+! links the shim's hybgen instead of hybgen.F90, does not need them.
+      module hybgen_ref_extras
+      use iso_c_binding
+      use mod_xc
+      use mod_cb_arrays
+      implicit none
+      contains
+
+! sig(t,s) of the reference equation of state (stmt_fns.h:505-511).
+      real(c_double) function href_sig(xt, xs) bind(C, name='href_sig')
+      real(c_double), value :: xt, xs
+#include "stmt_fns.h"
+      href_sig = sig(xt, xs)
+      end function href_sig
+
+! The reference's physical constants (mod_cb_arrays.F90:843-857) and the
+! folded EOS coefficients (stmt_fns.h:251-262), for bitwise pinning.
+      subroutine href_consts(c) bind(C, name='href_consts')
+      real(c_double), intent(out) :: c(16)
+#include "stmt_fns.h"
+      c(1) = onem
+      c(2) = tenm
+      c(3) = onemm
+      c(4) = epsil
+      c(5) = qonem
+      c(6) = svref
+      c(7) = c101
+      c(8) = c103
+      c(9) = c105
+      c(10) = c108
+      c(11) = c109
+      c(12) = c111
+      c(13) = rpdb
+      c(14) = 0.001*onemm
+      c(15) = 0.0
+      c(16) = 0.0
+      end subroutine href_consts
+
+! One column through the reference remapper pair, as hybgenaj calls them
+! (hybgen.F90:1530-1548): method 0 PCM, 1 PLM, 2 PPM, 3 WENO.
+      subroutine href_remap(method, kin, si, pin, po, lcm_i, thin, so) &
+          bind(C, name='href_remap')
+      integer(c_int), value :: method, kin
+      real(c_double), intent(in) :: si(kin), pin(kin+1), po(kin+1)
+      integer(c_int), intent(in) :: lcm_i(kin)
+      real(c_double), value :: thin
+      real(c_double), intent(out) :: so(kin)
+      real :: dprs(kin), dpi(kin), c1d(kin,1,3), s1(kin,1), f1(kin,1)
+      real :: pi1(kin+1), po1(kin+1), th
+      logical :: lcm(kin)
+      integer :: k
+      external hybgen_pcm_remap, hybgen_plm_coefs, hybgen_plm_remap, &
+               hybgen_ppm_coefs, hybgen_ppm_remap, &
+               hybgen_weno_coefs, hybgen_weno_remap
+      th = thin
+      do k= 1,kin
+        s1(k,1) = si(k)
+        lcm(k) = lcm_i(k) .ne. 0
+      enddo
+      do k= 1,kin+1
+        pi1(k) = pin(k)
+        po1(k) = po(k)
+      enddo
+      do k= 1,kin
+        dprs(k) = pi1(k+1)-pi1(k)
+        dpi(k)  = max(dprs(k),th)
+      enddo
+      if     (method.eq.0) then
+        call hybgen_pcm_remap(s1,pi1,dprs,f1,po1,kin,kin,1,th)
+      elseif (method.eq.1) then
+        call hybgen_plm_coefs(s1,dprs,lcm,c1d,kin,1,th)
+        call hybgen_plm_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
+      elseif (method.eq.2) then
+        call hybgen_ppm_coefs(s1,dpi,lcm,c1d,kin,1,th)
+        call hybgen_ppm_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
+      else
+        call hybgen_weno_coefs(s1,dpi,lcm,c1d,kin,1,th)
+        call hybgen_weno_remap(s1,pi1,dprs,c1d,f1,po1,kin,kin,1,th)
+      endif
+      do k= 1,kin
+        so(k) = f1(k,1)
+      enddo
+      end subroutine href_remap
+
+      end module hybgen_ref_extras
