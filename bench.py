@@ -80,6 +80,49 @@ def cpu_baseline(workload, hybmap, rows, seed):
     }
 
 
+def mixed_leg(lib, dd, S, sea, args, stream):
+    """The same workload in HYB_PREC_MIXED (float reconstruction and piece
+    integrals, FP64 interfaces and sums): throughput on the full domain, and
+    the error against the FP64 path on a 600x400 window of the same field."""
+    import torch
+    P = dd.P
+    P.precision = abi.PREC_MIXED
+    dd.synth(S, stream=stream)
+    for _ in range(args.warmup):
+        dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    P.precision = abi.PREC_FP64
+    print(f"[bench] mixed {1e3 * t:.2f} ms/step", file=sys.stderr, flush=True)
+    # error on a window: FP64 and mixed from the same input
+    w0 = abi.WORKLOADS[args.workload]
+    Pw, ww = abi.workload_params(args.workload, hybmap=args.hybmap, idm=600, jdm=400)
+    Sw = abi.make_synth(Pw, ww, seed=args.seed, itdm=w0["idm"], jtdm=w0["jdm"],
+                        i0=w0["idm"] // 2 - 300, j0=w0["jdm"] // 2 - 200)
+    a = abi.DeviceDomain(Pw).synth(Sw)
+    Pm, _ = abi.workload_params(args.workload, hybmap=args.hybmap, idm=600, jdm=400)
+    Pm.precision = abi.PREC_MIXED
+    b = abi.DeviceDomain(Pm).synth(Sw)
+    a.hybgen(m=1, n=2)
+    b.hybgen(m=1, n=2)
+    torch.cuda.synchronize()
+    ha, hb = a.download(), b.download()
+    err = {}
+    for f in ("temp", "saln", "th3d", "tracer", "u", "v"):
+        x, y = ha.a[f], hb.a[f]
+        err[f] = float(abs(x - y).max() / max(abs(x).max(), 1e-300))
+    same = all((ha.a[f] == hb.a[f]).all() for f in ("p", "dp", "dpu", "dpv", "pu", "pv"))
+    return {"value": sea * P.kdm / t, "unit": "column-layers/s", "ms_per_step": 1e3 * t,
+            "dtype": "f32 reconstruction, f64 interfaces and sums",
+            "speedup_vs_fp64": None, "max_rel_error_vs_fp64": err,
+            "interfaces_bitwise_fp64": bool(same),
+            "error_window": "600x400 centre window of the same field"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +133,8 @@ def main():
     ap.add_argument("--seed", type=int, default=200)
     ap.add_argument("--cpu-rows", type=int, default=160)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision (HYB_PREC_MIXED) leg")
     ap.add_argument("--kernel-path", type=int, default=0,
                     help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
     ap.add_argument("--remap-variant", type=int, default=None,
@@ -156,6 +201,10 @@ def main():
     p50 = srt[len(srt) // 2] * 1e3
     p99 = srt[min(len(srt) - 1, int(round(0.99 * (len(srt) - 1))))] * 1e3
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(lib, dd, S, sea, args, stream)
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_hybmap{args.hybmap}.json")
     if os.path.exists(tf):
@@ -203,7 +252,10 @@ def main():
         "call_bytes_per_sea_column": bytes_per_column_call(P.kdm, P.ntracr),
         "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
+    if mixed:
+        mixed["speedup_vs_fp64"] = mixed["value"] / value
     print(json.dumps(out), flush=True)
 
 

@@ -339,8 +339,10 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   if ((rc = check_fields(P, F, raflag))) return rc;
   if (m < 1 || m > 2 || n < 1 || n > 2 || (raflag && m == n))
     return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
-  if (P->precision != HYB_PREC_FP64)
-    return fail(HYB_EUNSUPPORTED, "mixed precision path not built yet");
+  if (P->precision == HYB_PREC_MIXED && !stream_path_ok(kparams_of(P)))
+    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm and no HYBGEN_DENFIX");
+  if (P->precision == HYB_PREC_MIXED && (g_kernel_path == 1 || g_kernel_path == 2))
+    return fail(HYB_EUNSUPPORTED, "the mixed form runs on the streaming kernels only");
   if (P->check_finite) return fail(HYB_EUNSUPPORTED, "finite scan not built yet");
   if ((rc = device_ready())) return rc;
   KParams K = kparams_of(P);

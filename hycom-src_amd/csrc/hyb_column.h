@@ -18,6 +18,7 @@ namespace hyb {
 struct KParams {
   int32_t ii, jj, kk, nhybrd, ntracr, hybmap, isopcm, ldenfix, nreg;
   int32_t itr801, itr802, itr803;
+  int32_t precision;  // HYB_PREC_FP64 or HYB_PREC_MIXED (remap arithmetic)
   double thbase, hybrlx, hybiso, hybthn, hybthk, thkbop, dp00i, dpns, dsns;
   double ra2fac, vland, diag_q801, diag_q80x;
   double dp0k[64], ds0k[64], dx0k[64];

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch 
   regrid_stream(P, V, W, i, j, n);
 }
 
-template <int METHOD, int PD>
+template <int METHOD, int PD, bool MIXED = false>
 __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, int nl) {
   extern __shared__ double lds[];
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, in
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
   LaneRow dcol{lds + threadIdx.x, kWave};
-  const bool ok = uv_stream<METHOD, PD>(P, V, i, j, nl, isv, dcol);
+  const bool ok = uv_stream<METHOD, PD, MIXED>(P, V, i, j, nl, isv, dcol);
   W.uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
 }
 
@@ -34,7 +34,11 @@ static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, 
   dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
   const size_t lds = sizeof(double) * P.kk * kWave;
   const int m = uv_method(P);
-  if (pd > 0) {
+  if (P.precision == 1) {
+    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 0, true>), gw3, w1, lds, s, P, V, W, nl);
+    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 0, true>), gw3, w1, lds, s, P, V, W, nl);
+    else hipLaunchKernelGGL((k_uv_s<3, 0, true>), gw3, w1, lds, s, P, V, W, nl);
+  } else if (pd > 0) {
     if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 2>), gw3, w1, lds, s, P, V, W, nl);
     else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 2>), gw3, w1, lds, s, P, V, W, nl);
     else hipLaunchKernelGGL((k_uv_s<3, 2>), gw3, w1, lds, s, P, V, W, nl);

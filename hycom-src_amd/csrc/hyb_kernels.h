@@ -36,8 +36,8 @@ bool stream_path_ok(const KParams& P);
 // remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
 // uv_prefetch the prefetch depth of the u/v walk (0 or 2).
 struct StreamTuning {
-  int remap_variant = 0;
-  int uv_prefetch = 0;
+  int remap_variant = 2;  // 3 fields per walk, no prefetch (fastest measured)
+  int uv_prefetch = 2;
 };
 // Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
 // plus uvf (2*NIJ int32).

@@ -24,6 +24,7 @@ inline KParams kparams_of(const hyb_params* P) {
   K.itr801 = P->itracr801;
   K.itr802 = P->itracr802;
   K.itr803 = P->itracr803;
+  K.precision = P->precision;
   K.thbase = P->thbase;
   K.hybrlx = 1.0 / P->qhybrlx;             // hybgen.F90:376
   K.hybiso = P->hybiso;

@@ -719,6 +719,220 @@ struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
 };
 
 // ---------------------------------------------------------------------
+// Reconstruction coefficients of old layer l for NF fields, in arithmetic
+// type R (double: the reference's operations; float: the mixed form).
+// Windows s0..s3 = s(l-1..l+2); dprs_* / dpi_* are the thicknesses of
+// layers l-1, l, l+1, l+2 (dpi = max(dprs, thin)).  Carried state in st.
+// ---------------------------------------------------------------------
+template <class R, int NF>
+struct CoefState {
+  R asc[NF], alc[NF];                     // PPM: as(l), raw al(l)
+  R wc1[NF], wc2[NF], wz1[NF], wz2[NF];   // WENO first sweep of layer l
+  R zw3l[NF];                             // WENO interface value zw3(l)
+  R qdpjp_l, wq_l;                        // PPM qdpjp(l); WENO 1/(dp(l)+dp(l+1))
+};
+
+template <class R>
+HYB_HD R gmax(R a, R b) { return a > b ? a : b; }
+template <class R>
+HYB_HD R gmin(R a, R b) { return a < b ? a : b; }
+template <class R>
+HYB_HD R gmin3(R a, R b, R c) { return gmin(gmin(a, b), c); }
+template <class R>
+HYB_HD R gabs(R a) { return a < R(0) ? -a : a; }
+template <class R>
+HYB_HD R gsign(R a, R b) {  // Fortran sign(a,b)
+  R x = gabs(a);
+  return signbit(b) ? -x : x;
+}
+
+template <int METHOD, int NF, class R, class LCMB>
+__attribute__((always_inline)) HYB_HD void coef_layer(
+    int l, int K, int nf, const LCMB& lcmb, bool lc_l, R thin, R dsmll, R dprs_m, R dprs_0,
+    R dprs_p, R dpi_m, R dpi_0, R dpi_p, R dpi_p2, const R* s0, const R* s1, const R* s2,
+    const R* s3, CoefState<R, NF>& st, R* c1l, R* c2l, R* c3l) {
+  if (METHOD == 1) {
+    // hybgen_plm_coefs (2393-2458) on dprs
+    const bool flat = (l == 1 || l == K || lc_l || dprs_0 <= thin);
+    R qcen = R(0);
+    if (!flat) qcen = dprs_0 / (dprs_0 + R(0.5) * (dprs_m + dprs_p));
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      c1l[f] = R(0);
+      c2l[f] = R(0);
+      c3l[f] = R(0);
+      if (f < nf && !flat) {
+        R ztop = R(2) * (s1[f] - s0[f]);
+        R zbot = R(2) * (s2[f] - s1[f]);
+        R zcen = qcen * (s2[f] - s0[f]);
+        if (ztop * zbot > R(0)) c1l[f] = gsign(gmin3(gabs(zcen), gabs(zbot), gabs(ztop)), zbot);
+      }
+    }
+  } else if (METHOD == 2) {
+    // hybgen_ppm_coefs (2587-2706) on dpi: as(l+1), al(l+1), then the
+    // limited edge values of layer l
+    const int jn = l + 1;
+    const bool do_as = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
+    const bool do_al = jn >= 3 && jn <= K - 1;
+    R qdpjp_n = R(0), dpq3 = R(0), dp2jp_l = R(0), dpj2p_n = R(0);
+    R dpjp_lm = R(0), dpjp_n = R(0), qdp2jp_l = R(0), qdpj2p_l = R(0), qdp4 = R(0);
+    if (jn <= K - 1) {
+      const R dpjp_l = dpi_0 + dpi_p;
+      dpjp_n = dpi_p + dpi_p2;
+      qdpjp_n = R(1) / dpjp_n;  // qdpjp(l+1)
+      dp2jp_l = dpi_0 + dpjp_l;
+      dpj2p_n = dpjp_n + dpi_p2;
+      dpq3 = dpi_p / (dpi_0 + dpjp_n);
+      if (do_al) {
+        dpjp_lm = dpi_m + dpi_0;
+        const R dpj2p_l = dpjp_l + dpi_p;
+        qdp2jp_l = R(1) / dp2jp_l;
+        qdpj2p_l = R(1) / dpj2p_l;
+        qdp4 = R(1) / (dpjp_lm + dpjp_n);
+      }
+    }
+    const bool lim = l >= 2 && l <= K - 1;
+    const bool lim_flat = lim && (lc_l || dpi_0 <= thin);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      c1l[f] = R(0);
+      c2l[f] = R(0);
+      c3l[f] = R(0);
+      if (f < nf) {
+        R as_n = R(0), al_n = R(0);
+        if (do_as) {
+          const R slj = s2[f] - s1[f];
+          const R srj = s3[f] - s2[f];
+          if (slj * srj > R(0)) {
+            R scj = dpq3 * (dp2jp_l * srj * qdpjp_n + dpj2p_n * slj * st.qdpjp_l);
+            as_n = gsign(gmin3(gabs(R(2) * slj), gabs(scj), gabs(R(2) * srj)), scj);
+          }
+        }
+        if (jn == K) {
+          al_n = s2[f];  // al(kk) = s(kk)
+        } else if (jn == 2) {
+          al_n = s1[f];  // al(2) = s(1)
+        } else if (do_al) {
+          al_n = s1[f] + dpi_0 * (s2[f] - s1[f]) * st.qdpjp_l +
+                 qdp4 * (R(2) * dpi_p * dpi_0 * st.qdpjp_l * (s2[f] - s1[f]) *
+                             (dpjp_lm * qdp2jp_l - dpjp_n * qdpj2p_l) -
+                         dpi_0 * as_n * dpjp_lm * qdp2jp_l + dpi_p * st.asc[f] * dpjp_n * qdpj2p_l);
+        }
+        R al = st.alc[f], ar;
+        if (l == K - 1) ar = s2[f];  // ar(kk-1) = s(kk)
+        else if (l == K) ar = s1[f];  // ar(kk) = s(kk)
+        else if (l == 1) ar = s1[f];  // ar(1) = s(1)
+        else ar = al_n;               // ar(l) = al(l+1)
+        if (lim) {
+          if (lim_flat) {
+            al = s1[f];
+            ar = s1[f];
+          } else if ((s2[f] - s1[f]) * (s1[f] - s0[f]) <= R(0)) {
+            al = s1[f];
+            ar = s1[f];
+          } else {
+            const R da = ar - al;
+            const R a6 = R(6) * s1[f] - R(3) * (al + ar);
+            if (da * a6 > da * da) {
+              al = R(3) * s1[f] - R(2) * ar;
+            } else if (da * a6 < -(da * da)) {
+              ar = R(3) * s1[f] - R(2) * al;
+            }
+          }
+        }
+        if (al != ar) {
+          c1l[f] = al;
+          c2l[f] = ar - al;
+          c3l[f] = R(6) * s1[f] - R(3) * (al + ar);
+        } else {
+          c1l[f] = al;
+        }
+        st.asc[f] = as_n;
+        st.alc[f] = al_n;
+      }
+    }
+    st.qdpjp_l = qdpjp_n;
+  } else if (METHOD == 3) {
+    // hybgen_weno_coefs (2845-2968) on dpi: first sweep of layer l+1,
+    // interface value zw3(l+1), second sweep of layer l
+    const int jn = l + 1;
+    const bool first_full = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
+    R wq_n = R(0), qmjp = R(0), dpjm2jp = R(0);
+    if (jn <= K - 1) wq_n = R(1) / (dpi_p + dpi_p2);  // qdpjm(l+2)
+    if (first_full) {
+      qmjp = R(1) / (dpi_0 + dpi_p + dpi_p2);
+      dpjm2jp = dpi_0 + R(2) * dpi_p + dpi_p2;
+    }
+    const bool second = l >= 2 && l <= K - 1 && !(lc_l || dpi_0 <= thin);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      c1l[f] = R(0);
+      c2l[f] = R(0);
+      c3l[f] = R(0);
+      if (f < nf) {
+        R n1 = s2[f], n2 = s2[f], nz1 = R(0), nz2 = R(0);
+        if (first_full) {
+          const R zw3_j = st.wq_l * (s2[f] - s1[f]);
+          const R zw3_jp = wq_n * (s3[f] - s2[f]);
+          R q001 = dpi_p * zw3_jp;
+          R q002 = dpi_p * zw3_j;
+          if (q001 * q002 < R(0)) {
+            q001 = R(0);
+            q002 = R(0);
+          }
+          const R q01 = dpjm2jp * zw3_jp;
+          const R q02 = dpjm2jp * zw3_j;
+          if (gabs(q001) > gabs(q02)) q001 = q02;
+          if (gabs(q002) > gabs(q01)) q002 = q01;
+          const R qq = (q001 - q002) * qmjp;
+          q001 = q001 - qq * dpi_p2;
+          q002 = q002 + qq * dpi_0;
+          n2 = s2[f] + q001;
+          n1 = s2[f] - q002;
+          const R a = R(2) * q001 - q002, b = R(2) * q002 - q001;
+          nz1 = a * a;
+          nz2 = b * b;
+        }
+        R zw3n = R(0);
+        if (jn <= K) {
+          const R q002 = gmax(st.wz2[f], dsmll);
+          const R q001 = gmax(nz1, dsmll);
+          zw3n = (q001 * st.wc2[f] + q002 * n1) / (q001 + q002);
+        }
+        c1l[f] = st.wc1[f];
+        c2l[f] = st.wc2[f];
+        if (second) {
+          R q01 = zw3n - s1[f];
+          R q02 = s1[f] - st.zw3l[f];
+          const R q001 = R(2) * q01;
+          const R q002 = R(2) * q02;
+          if (q01 * q02 < R(0)) {
+            q01 = R(0);
+            q02 = R(0);
+          } else if (gabs(q01) > gabs(q002)) {
+            q01 = q002;
+          } else if (gabs(q02) > gabs(q001)) {
+            q02 = q001;
+          }
+          c1l[f] = s1[f] - q02;
+          c2l[f] = s1[f] + q01;
+        }
+        st.zw3l[f] = zw3n;
+        st.wc1[f] = n1;
+        st.wc2[f] = n2;
+        st.wz1[f] = nz1;
+        st.wz2[f] = nz2;
+      }
+    }
+    st.wq_l = wq_n;
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; f++) c1l[f] = c2l[f] = c3l[f] = R(0);
+  }
+
+}
+
+// ---------------------------------------------------------------------
 // remap of up to NF fields of one column in one walk over the old layers
 // (hybgen_{pcm,plm,ppm,weno}_{coefs,remap}).  fp[f] points at level 1 of
 // field f (stride S); levels > kp take level kp's value (the massless
@@ -737,9 +951,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1), s(l), s(l+1), s(l+2)
   double smin[NF], smax[NF], skp[NF], sfirst[NF];
   double acc[NF], ov[NF], prev[NF];
-  double asc[NF], alc[NF];                     // PPM: as(l), raw al(l)
-  double wc1[NF], wc2[NF], wz1[NF], wz2[NF];   // WENO first sweep of layer l
-  double zw3l[NF];                             // WENO interface value zw3(l)
+  CoefState<double, NF> cst;                   // reconstruction state
   double sq[NF][PQ];                           // s(l+3..l+2+PD), loaded ahead
   double prq[PQ];                              // raw(l+3..l+2+PD) of pi
   int xl = 4;                                  // next level to load
@@ -767,8 +979,8 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
     s0[f] = s1[f] = s2[f] = s3[f] = 0.0;
     smin[f] = smax[f] = skp[f] = sfirst[f] = 0.0;
     acc[f] = ov[f] = prev[f] = 0.0;
-    asc[f] = alc[f] = 0.0;
-    wc1[f] = wc2[f] = wz1[f] = wz2[f] = zw3l[f] = 0.0;
+    cst.asc[f] = cst.alc[f] = 0.0;
+    cst.wc1[f] = cst.wc2[f] = cst.wz1[f] = cst.wz2[f] = cst.zw3l[f] = 0.0;
     if (f < nf) {
       s1[f] = load(f, 1);
       if (K >= 2) s2[f] = load(f, 2);
@@ -776,10 +988,10 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
       sfirst[f] = s1[f];
 #pragma unroll
       for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? load(f, 4 + q) : 0.0;
-      if (METHOD == 2) alc[f] = s1[f];  // al(1) = s(1)
-      if (METHOD == 3) {                // first sweep of layer 1
-        wc1[f] = s1[f];
-        wc2[f] = s1[f];
+      if (METHOD == 2) cst.alc[f] = s1[f];  // al(1) = s(1)
+      if (METHOD == 3) {                    // first sweep of layer 1
+        cst.wc1[f] = s1[f];
+        cst.wc2[f] = s1[f];
       }
     }
   }
@@ -797,11 +1009,12 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   if (K >= 2 && dmax(pv3 - pv2, thin) > thin) thick |= 2ull;
   if (K >= 3 && dmax(pv4 - pv3, thin) > thin) thick |= 4ull;
   // carried metrics: PPM qdpjp(l); WENO 1/(dp(l)+dp(l+1))
-  double qdpjp_l = 0.0, wq_l = 0.0;
+  cst.qdpjp_l = 0.0;
+  cst.wq_l = 0.0;
   if (K >= 2) {
     const double d1 = dmax(pv2 - pv1, thin), d2 = dmax(pv3 - pv2, thin);
-    qdpjp_l = 1.0 / (d1 + d2);
-    wq_l = 1.0 / (d1 + d2);
+    cst.qdpjp_l = 1.0 / (d1 + d2);
+    cst.wq_l = 1.0 / (d1 + d2);
   }
 
   // walk state (2729-2735)
@@ -844,184 +1057,9 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
     const bool lc_l = lcmb(l);
     // ------------------------------------------ coefficients of layer l
     double c1l[NF], c2l[NF], c3l[NF];
-    if (METHOD == 1) {
-      // hybgen_plm_coefs (2393-2458) on dprs
-      const bool flat = (l == 1 || l == K || lc_l || dprs_0 <= thin);
-      double qcen = 0.0;
-      if (!flat) qcen = dprs_0 / (dprs_0 + 0.5 * (dprs_m + dprs_p));
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        c1l[f] = 0.0;
-        c2l[f] = 0.0;
-        c3l[f] = 0.0;
-        if (f < nf && !flat) {
-          double ztop = 2.0 * (s1[f] - s0[f]);
-          double zbot = 2.0 * (s2[f] - s1[f]);
-          double zcen = qcen * (s2[f] - s0[f]);
-          if (ztop * zbot > 0.0) c1l[f] = dsign(dmin3(dabs(zcen), dabs(zbot), dabs(ztop)), zbot);
-        }
-      }
-    } else if (METHOD == 2) {
-      // hybgen_ppm_coefs (2587-2706) on dpi: as(l+1), al(l+1), then the
-      // limited edge values of layer l
-      const int jn = l + 1;
-      const bool do_as = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
-      const bool do_al = jn >= 3 && jn <= K - 1;
-      double qdpjp_n = 0.0, dpq3 = 0.0, dp2jp_l = 0.0, dpj2p_n = 0.0;
-      double dpjp_lm = 0.0, dpjp_n = 0.0, qdp2jp_l = 0.0, qdpj2p_l = 0.0, qdp4 = 0.0;
-      if (jn <= K - 1) {
-        const double dpjp_l = dpi_0 + dpi_p;
-        dpjp_n = dpi_p + dpi_p2;
-        qdpjp_n = 1.0 / dpjp_n;  // qdpjp(l+1)
-        dp2jp_l = dpi_0 + dpjp_l;
-        dpj2p_n = dpjp_n + dpi_p2;
-        dpq3 = dpi_p / (dpi_0 + dpjp_n);
-        if (do_al) {
-          dpjp_lm = dpi_m + dpi_0;
-          const double dpj2p_l = dpjp_l + dpi_p;
-          qdp2jp_l = 1.0 / dp2jp_l;
-          qdpj2p_l = 1.0 / dpj2p_l;
-          qdp4 = 1.0 / (dpjp_lm + dpjp_n);
-        }
-      }
-      const bool lim = l >= 2 && l <= K - 1;
-      const bool lim_flat = lim && (lc_l || dpi_0 <= thin);
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        c1l[f] = 0.0;
-        c2l[f] = 0.0;
-        c3l[f] = 0.0;
-        if (f < nf) {
-          double as_n = 0.0, al_n = 0.0;
-          if (do_as) {
-            const double slj = s2[f] - s1[f];
-            const double srj = s3[f] - s2[f];
-            if (slj * srj > 0.0) {
-              double scj = dpq3 * (dp2jp_l * srj * qdpjp_n + dpj2p_n * slj * qdpjp_l);
-              as_n = dsign(dmin3(dabs(2.0 * slj), dabs(scj), dabs(2.0 * srj)), scj);
-            }
-          }
-          if (jn == K) {
-            al_n = s2[f];  // al(kk) = s(kk)
-          } else if (jn == 2) {
-            al_n = s1[f];  // al(2) = s(1)
-          } else if (do_al) {
-            al_n = s1[f] + dpi_0 * (s2[f] - s1[f]) * qdpjp_l +
-                   qdp4 * (2.0 * dpi_p * dpi_0 * qdpjp_l * (s2[f] - s1[f]) *
-                               (dpjp_lm * qdp2jp_l - dpjp_n * qdpj2p_l) -
-                           dpi_0 * as_n * dpjp_lm * qdp2jp_l + dpi_p * asc[f] * dpjp_n * qdpj2p_l);
-          }
-          double al = alc[f], ar;
-          if (l == K - 1) ar = s2[f];  // ar(kk-1) = s(kk)
-          else if (l == K) ar = s1[f];  // ar(kk) = s(kk)
-          else if (l == 1) ar = s1[f];  // ar(1) = s(1)
-          else ar = al_n;               // ar(l) = al(l+1)
-          if (lim) {
-            if (lim_flat) {
-              al = s1[f];
-              ar = s1[f];
-            } else if ((s2[f] - s1[f]) * (s1[f] - s0[f]) <= 0.0) {
-              al = s1[f];
-              ar = s1[f];
-            } else {
-              const double da = ar - al;
-              const double a6 = 6.0 * s1[f] - 3.0 * (al + ar);
-              if (da * a6 > da * da) {
-                al = 3.0 * s1[f] - 2.0 * ar;
-              } else if (da * a6 < -(da * da)) {
-                ar = 3.0 * s1[f] - 2.0 * al;
-              }
-            }
-          }
-          if (al != ar) {
-            c1l[f] = al;
-            c2l[f] = ar - al;
-            c3l[f] = 6.0 * s1[f] - 3.0 * (al + ar);
-          } else {
-            c1l[f] = al;
-          }
-          asc[f] = as_n;
-          alc[f] = al_n;
-        }
-      }
-      qdpjp_l = qdpjp_n;
-    } else if (METHOD == 3) {
-      // hybgen_weno_coefs (2845-2968) on dpi: first sweep of layer l+1,
-      // interface value zw3(l+1), second sweep of layer l
-      const int jn = l + 1;
-      const bool first_full = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
-      double wq_n = 0.0, qmjp = 0.0, dpjm2jp = 0.0;
-      if (jn <= K - 1) wq_n = 1.0 / (dpi_p + dpi_p2);  // qdpjm(l+2)
-      if (first_full) {
-        qmjp = 1.0 / (dpi_0 + dpi_p + dpi_p2);
-        dpjm2jp = dpi_0 + 2.0 * dpi_p + dpi_p2;
-      }
-      const bool second = l >= 2 && l <= K - 1 && !(lc_l || dpi_0 <= thin);
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        c1l[f] = 0.0;
-        c2l[f] = 0.0;
-        c3l[f] = 0.0;
-        if (f < nf) {
-          double n1 = s2[f], n2 = s2[f], nz1 = 0.0, nz2 = 0.0;
-          if (first_full) {
-            const double zw3_j = wq_l * (s2[f] - s1[f]);
-            const double zw3_jp = wq_n * (s3[f] - s2[f]);
-            double q001 = dpi_p * zw3_jp;
-            double q002 = dpi_p * zw3_j;
-            if (q001 * q002 < 0.0) {
-              q001 = 0.0;
-              q002 = 0.0;
-            }
-            const double q01 = dpjm2jp * zw3_jp;
-            const double q02 = dpjm2jp * zw3_j;
-            if (dabs(q001) > dabs(q02)) q001 = q02;
-            if (dabs(q002) > dabs(q01)) q002 = q01;
-            const double qq = (q001 - q002) * qmjp;
-            q001 = q001 - qq * dpi_p2;
-            q002 = q002 + qq * dpi_0;
-            n2 = s2[f] + q001;
-            n1 = s2[f] - q002;
-            const double a = 2.0 * q001 - q002, b = 2.0 * q002 - q001;
-            nz1 = a * a;
-            nz2 = b * b;
-          }
-          double zw3n = 0.0;
-          if (jn <= K) {
-            const double q002 = dmax(wz2[f], dsmll);
-            const double q001 = dmax(nz1, dsmll);
-            zw3n = (q001 * wc2[f] + q002 * n1) / (q001 + q002);
-          }
-          c1l[f] = wc1[f];
-          c2l[f] = wc2[f];
-          if (second) {
-            double q01 = zw3n - s1[f];
-            double q02 = s1[f] - zw3l[f];
-            const double q001 = 2.0 * q01;
-            const double q002 = 2.0 * q02;
-            if (q01 * q02 < 0.0) {
-              q01 = 0.0;
-              q02 = 0.0;
-            } else if (dabs(q01) > dabs(q002)) {
-              q01 = q002;
-            } else if (dabs(q02) > dabs(q001)) {
-              q02 = q001;
-            }
-            c1l[f] = s1[f] - q02;
-            c2l[f] = s1[f] + q01;
-          }
-          zw3l[f] = zw3n;
-          wc1[f] = n1;
-          wc2[f] = n2;
-          wz1[f] = nz1;
-          wz2[f] = nz2;
-        }
-      }
-      wq_l = wq_n;
-    } else {
-#pragma unroll
-      for (int f = 0; f < NF; f++) c1l[f] = c2l[f] = c3l[f] = 0.0;
-    }
+    coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, dprs_m, dprs_0, dprs_p,
+                                   dpi_m, dpi_0, dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l, c2l,
+                                   c3l);
 
     const double pil = pv1, pil1 = pv2, dprsl = dprs_0;
     // Every output still to come ends in this layer when it reaches the
@@ -1244,6 +1282,269 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   sink.fixup(smin, smax);
 }
 
+// ---------------------------------------------------------------------
+// Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart).  The
+// walk is the same, but:
+//  * the reconstruction coefficients and their metrics are float
+//    (coef_layer<float>);
+//  * each old layer l is cut by the new interfaces into pieces; a piece
+//    [za,zb] carries s(l)*(zb-za) in FP64 plus (zb-za) times the average
+//    of the reconstruction's deviation from s(l) over the piece, evaluated
+//    in float in factored form, so thin pieces lose no precision;
+//  * the float rounding of a layer (its deviations sum to zero exactly)
+//    is returned to the layer's pieces in proportion to their widths, so
+//    every old layer hands out exactly s(l)*dp(l) in FP64 and the column
+//    integral is conserved to FP64 roundoff (absent thin outputs and
+//    clamping, as in the reference);
+//  * an output is its FP64 piece sum divided by its thickness (FP64).
+// The overturn/unmix, the interfaces and every decision of the regrid stay
+// FP64 and bitwise the reference's.
+// ---------------------------------------------------------------------
+template <int METHOD, int NF>
+__attribute__((always_inline)) HYB_HD void avgdev_f(const float* c1, const float* c2,
+                                                    const float* c3, const float* sf, float xa,
+                                                    float xe, float* out) {
+  // average over [xa,xe] of r(x) - s for the reconstruction r of one layer
+  const float sum = xa + xe;
+  const float q2 = xa * xa + xa * xe + xe * xe;  // 3 * average of x^2
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    if (METHOD == 0) {
+      out[f] = 0.0f;
+    } else if (METHOD == 1) {  // r = s + c1 (x - 1/2)
+      out[f] = c1[f] * (0.5f * sum - 0.5f);
+    } else if (METHOD == 2) {  // r = c1 + (c2 + c3) x - c3 x^2
+      out[f] = (c1[f] - sf[f]) + (c2[f] + c3[f]) * (0.5f * sum) - c3[f] * (q2 * (1.0f / 3.0f));
+    } else {  // r = s + (c1 - s)(3x^2 - 4x + 1) + (c2 - s)(3x^2 - 2x)
+      out[f] = (c1[f] - sf[f]) * (q2 - 2.0f * sum + 1.0f) + (c2[f] - sf[f]) * (q2 - sum);
+    }
+  }
+}
+
+template <int METHOD, int NF, class PI, class PO, class SINK>
+HYB_HD void walk_mixed(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
+                       int nf, int kp, uint64_t lcm, int64_t S) {
+  const double thin = kDpthin;
+  const float thinf = (float)kDpthin;
+  auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
+  double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1..l+2) as stored
+  double smin[NF], smax[NF], skp[NF], sfirst[NF], prev[NF], acc[NF];
+  CoefState<float, NF> cst;
+  int xl = 4;
+  auto load = [&](int f, int x) -> double {
+    double v;
+    if (x <= kp) {
+      v = fp[f][(int64_t)(x - 1) * S];
+    } else {
+      v = skp[f];
+    }
+    if (x == kp) skp[f] = v;
+    if (x == 1) {
+      smin[f] = v;
+      smax[f] = v;
+    } else {
+      smin[f] = dmin(smin[f], v);
+      smax[f] = dmax(smax[f], v);
+    }
+    return v;
+  };
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    s0[f] = s1[f] = s2[f] = s3[f] = 0.0;
+    smin[f] = smax[f] = skp[f] = sfirst[f] = prev[f] = acc[f] = 0.0;
+    cst.asc[f] = cst.alc[f] = 0.0f;
+    cst.wc1[f] = cst.wc2[f] = cst.wz1[f] = cst.wz2[f] = cst.zw3l[f] = 0.0f;
+    if (f < nf) {
+      s1[f] = load(f, 1);
+      if (K >= 2) s2[f] = load(f, 2);
+      if (K >= 3) s3[f] = load(f, 3);
+      sfirst[f] = s1[f];
+      if (METHOD == 2) cst.alc[f] = (float)s1[f];
+      if (METHOD == 3) {
+        cst.wc1[f] = (float)s1[f];
+        cst.wc2[f] = (float)s1[f];
+      }
+    }
+  }
+  double pv0 = 0.0, pv1 = pi.first(), pv2 = 0.0, pv3 = 0.0, pv4 = 0.0;
+  if (K + 1 >= 2) pv2 = pi.nextfrom(1, pv1);
+  if (K + 1 >= 3) pv3 = pi.nextfrom(2, pv2);
+  if (K + 1 >= 4) pv4 = pi.nextfrom(3, pv3);
+  uint64_t thick = 0;
+  if (dmax(pv2 - pv1, thin) > thin) thick |= 1ull;
+  if (K >= 2 && dmax(pv3 - pv2, thin) > thin) thick |= 2ull;
+  if (K >= 3 && dmax(pv4 - pv3, thin) > thin) thick |= 4ull;
+  cst.qdpjp_l = 0.0f;
+  cst.wq_l = 0.0f;
+  if (K >= 2) {
+    const float d1 = gmax((float)(pv2 - pv1), thinf), d2 = gmax((float)(pv3 - pv2), thinf);
+    cst.qdpjp_l = 1.0f / (d1 + d2);
+    cst.wq_l = 1.0f / (d1 + d2);
+  }
+  const double zb0 = dmax(po(1), pv1);
+  double zt_c = zb0;  // top of output kc
+  int kc = 1;
+  bool accon = false;  // output kc holds pieces of layers above
+
+  auto emit = [&](const double* v) {
+    bool certain = true;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+      if (f < nf) {
+        prev[f] = v[f];
+        if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
+      }
+    sink.emit(kc, v, certain, (thick >> (kc - 1)) & 1ull);
+    kc = kc + 1;
+  };
+
+#pragma unroll 1
+  for (int l = 1; l <= K; l++) {
+    const double dprs_m = pv1 - pv0, dprs_0 = pv2 - pv1, dprs_p = pv3 - pv2,
+                 dprs_p2 = pv4 - pv3;
+    const float fm = (float)dprs_m, f0 = (float)dprs_0, fp_ = (float)dprs_p,
+                fp2 = (float)dprs_p2;
+    float c1l[NF], c2l[NF], c3l[NF], sf0[NF], sf1[NF], sf2[NF], sf3[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      sf0[f] = (float)s0[f];
+      sf1[f] = (float)s1[f];
+      sf2[f] = (float)s2[f];
+      sf3[f] = (float)s3[f];
+    }
+    coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lcmb(l), thinf, 1.0e-8f, fm, f0, fp_,
+                                  gmax(fm, thinf), gmax(f0, thinf), gmax(fp_, thinf),
+                                  gmax(fp2, thinf), sf0, sf1, sf2, sf3, cst, c1l, c2l, c3l);
+    const double a = pv1, b = pv2;
+    if (l < K && b >= zx) {  // the walk ends in this layer: complete the range
+      for (int x = xl; x <= K; x++) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+          if (f < nf) (void)load(f, x);
+      }
+      xl = K + 1;
+    }
+    const bool active = kc <= K && (accon || zt_c < b || l == K || b >= zx);
+    if (active) {
+      const double dpiv = dmax(b - a, thin);
+      const float rdp = 1.0f / (float)dpiv;
+      const float rw = (b - a > 0.0) ? 1.0f / (float)(b - a) : 0.0f;
+      // pass A: sum of the float deviation integrals of this layer's pieces
+      double esum[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) esum[f] = 0.0;
+      PO po0 = po;
+      {
+        double za = accon ? a : zt_c;
+        for (int kq = kc; kq <= K; kq++) {
+          const double zb = dmin(po(kq + 1), zx);
+          const bool ends = (l == K) || (b >= zb);
+          const double ze = ends ? dmax(zb, za) : b;
+          const double w = ze - za;
+          if (w > 0.0) {
+            float dv[NF];
+            avgdev_f<METHOD, NF>(c1l, c2l, c3l, sf1, (float)(za - a) * rdp,
+                                 (float)(ze - a) * rdp, dv);
+#pragma unroll
+            for (int f = 0; f < NF; f++) esum[f] = esum[f] + w * (double)dv[f];
+          }
+          if (!ends) break;
+          za = ze;
+          if (zb >= b && !(l == K || b >= zx)) break;
+        }
+      }
+      po = po0;
+      // pass B: the pieces, corrected so the layer hands out s(l)*dp(l)
+      double za = accon ? a : zt_c;
+      for (; kc <= K;) {
+        const double zt = zt_c;
+        const double zb = dmin(po(kc + 1), zx);
+        const bool ends = (l == K) || (b >= zb);
+        const double ze = ends ? dmax(zb, za) : b;
+        const double w = ze - za;
+        if (w > 0.0) {
+          float dv[NF];
+          avgdev_f<METHOD, NF>(c1l, c2l, c3l, sf1, (float)(za - a) * rdp, (float)(ze - a) * rdp,
+                               dv);
+          const float share = (float)w * rw;
+#pragma unroll
+          for (int f = 0; f < NF; f++)
+            if (f < nf)
+              acc[f] = acc[f] + (s1[f] * w + (w * (double)dv[f] - esum[f] * (double)share));
+        }
+        if (!ends) {
+          accon = true;
+          break;
+        }
+        // output kc is complete (hybgen_*_remap: thin outputs copy the
+        // one above, the first one copies s(1))
+        double v[NF];
+        const bool thn = (zb - zt <= thin) || (zt >= zx);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          v[f] = 0.0;
+          if (f < nf) v[f] = thn ? ((kc != 1) ? prev[f] : sfirst[f]) : acc[f] / (zb - zt);
+          acc[f] = 0.0;
+        }
+        accon = false;
+        zt_c = zb;
+        emit(v);
+        za = ze;
+        if (zb >= b && !(l == K || b >= zx)) break;
+      }
+    }
+    if (kc > K) break;
+    // slide
+    const double top = pv4;
+    pv0 = pv1;
+    pv1 = pv2;
+    pv2 = pv3;
+    pv3 = top;
+    pv4 = (l + 4 <= K + 1) ? pi.nextfrom(l + 3, top) : 0.0;
+    if (l + 3 <= K && dmax(pv4 - pv3, thin) > thin) thick |= 1ull << (l + 2);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      s0[f] = s1[f];
+      s1[f] = s2[f];
+      s2[f] = s3[f];
+      s3[f] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
+    }
+    xl++;
+  }
+  sink.fixup(smin, smax);
+}
+
+template <int METHOD, int NF, class PO>
+HYB_HD void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
+                        PO& po) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int kpw = W.kp[c0];
+  if (kpw & kUnsafe) return;
+  const int kp = kpw & 0xffff;
+  const uint64_t lcm = W.lcm[c0];
+  const double* pres = W.pres + c0;
+  const double zx = pres[(int64_t)K * S];
+  const int nfields = 2 + P.ntracr;
+  for (int g0 = 0; g0 < nfields; g0 += NF) {
+    const int nf = (nfields - g0 < NF) ? nfields - g0 : NF;
+    double* fp[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const int g = g0 + f;
+      if (f >= nf) fp[f] = nullptr;
+      else if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
+      else if (g == 1) fp[f] = V.saln + c0 + lev4(V, 1, n);
+      else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
+    }
+    double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
+    PiArray pi{pres, S};
+    FieldSink<NF> sink{fp, nf, th, P.thbase, S};
+    walk_mixed<METHOD, NF>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
+  }
+}
+
 template <int METHOD, int NF, int PD, class PO>
 HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
                          PO& po) {
@@ -1284,7 +1585,7 @@ HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int 
 // would overwrite a level not yet read (the test of regrid_stream), it
 // writes pu only and returns false; point_c then redoes the point.
 // ---------------------------------------------------------------------
-template <int METHOD, int PD>
+template <int METHOD, int PD, bool MIXED = false>
 HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bool isv,
                       const LaneRow& dcol) {
   const int K = P.kk;
@@ -1328,7 +1629,8 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   PoSum po{dcol};
   UvSink sink{uu, S, ub, pr, &po};
   double* fp[1] = {uu};
-  walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+  if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+  else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
 #pragma unroll 2
   for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
   return true;

@@ -196,6 +196,15 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
         continue;
       }
       for (int k = 1; k <= kk + 1; k++) row[k - 1] = V.p[c0 + (int64_t)(k - 1) * V.NIJ];
+      if (P.precision == 1) {
+        switch (P.hybmap) {
+          case 0: remap_mixed<0, 3>(P, V, W, i, j, n, po); break;
+          case 1: remap_mixed<1, 3>(P, V, W, i, j, n, po); break;
+          case 2: remap_mixed<2, 3>(P, V, W, i, j, n, po); break;
+          default: remap_mixed<3, 3>(P, V, W, i, j, n, po); break;
+        }
+        continue;
+      }
       switch (P.hybmap) {
         case 0: remap_stream<0, kNFMax, 0>(P, V, W, i, j, n, po); break;
         case 1: remap_stream<1, 3, 2>(P, V, W, i, j, n, po); break;
@@ -237,7 +246,13 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
         for (int i = 1; i <= ii; i++) {
           if (!(isv ? V.iv : V.iu)[ij_of(V, i, j)]) continue;
           bool ok;
-          switch (uv_method(P)) {
+          if (P.precision == 1) {
+            switch (uv_method(P)) {
+              case 0: ok = uv_stream<0, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 1: ok = uv_stream<1, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              default: ok = uv_stream<3, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
+            }
+          } else switch (uv_method(P)) {
             case 0: ok = uv_stream<0, 2>(P, V, i, j, nl, isv != 0, dcol); break;
             case 1: ok = uv_stream<1, 0>(P, V, i, j, nl, isv != 0, dcol); break;
             default: ok = uv_stream<3, 2>(P, V, i, j, nl, isv != 0, dcol); break;
@@ -265,3 +280,41 @@ extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F,
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }
 
 extern "C" int64_t hostcheck_stream_uv_fallback_points(void) { return g_uv_fallback; }
+
+// ---- one column through the streaming walks (column-level checks) ----
+template <bool MIXED, int M>
+static void column_walk(int kk, const double* si, const double* pi, const double* po,
+                        const int* lcm0, double* so) {
+  double s[kKMax + 2], pres[kKMax + 2], prsf[kKMax + 2];
+  uint64_t lcm = 0;
+  for (int k = 0; k < kk; k++) {
+    s[k] = si[k];
+    if (lcm0 && lcm0[k]) lcm |= 1ull << k;
+  }
+  for (int k = 0; k <= kk; k++) {
+    pres[k] = pi[k];
+    prsf[k] = po[k];
+  }
+  double* fp[1] = {s};
+  double* op[1] = {so};  // outputs to their own array: no in-place hazard
+  PiArray pa{pres, 1};
+  LaneRow pr{prsf, 1};
+  FieldSink<1> sink{op, 1, nullptr, 0.0, 1};
+  if (MIXED) walk_mixed<M, 1>(kk, pres[kk], pa, pr, sink, fp, 1, kk, lcm, 1);
+  else walk_core<M, 1, 0>(kk, pres[kk], pa, pr, sink, fp, 1, kk, lcm, 1);
+}
+
+extern "C" void hostcheck_remap_column(int mixed, int method, int kk, const double* si,
+                                       const double* pi, const double* po, const int* lcm0,
+                                       double* so) {
+  switch (method + 4 * (mixed != 0)) {
+    case 0: column_walk<false, 0>(kk, si, pi, po, lcm0, so); break;
+    case 1: column_walk<false, 1>(kk, si, pi, po, lcm0, so); break;
+    case 2: column_walk<false, 2>(kk, si, pi, po, lcm0, so); break;
+    case 3: column_walk<false, 3>(kk, si, pi, po, lcm0, so); break;
+    case 4: column_walk<true, 0>(kk, si, pi, po, lcm0, so); break;
+    case 5: column_walk<true, 1>(kk, si, pi, po, lcm0, so); break;
+    case 6: column_walk<true, 2>(kk, si, pi, po, lcm0, so); break;
+    default: column_walk<true, 3>(kk, si, pi, po, lcm0, so); break;
+  }
+}

@@ -151,3 +151,28 @@ def test_streaming_any_kdm(gpu, kdm, hybmap, ntracr, extreme):
     got = dd.download()
     abi.check(abi.load().hycom_hybgen_set_kernel_path(0))
     _assert_same(want, got, f"kdm={kdm} hybmap={hybmap} ntracr={ntracr}")
+
+
+@pytest.mark.parametrize("hybmap", [1, 2, 3])
+def test_mixed_precision_on_gpu(gpu, hybmap):
+    """HYB_PREC_MIXED on the GPU: interfaces bitwise the FP64 oracle's, every
+    remapped field within 1e-5 of its scale, and the same bits as the host
+    build of the same mixed code."""
+    import hostcheck_lib as hc
+    P, w = _case("mixed", idm=90, jdm=57, kdm=41, hybmap=hybmap, ntracr=3, trcflg=[2, 0, 0])
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=9, extreme=1))
+    want = _oracle_out(d0)
+    P.precision = abi.PREC_MIXED
+    host = d0.copy()
+    host.P = P
+    hc.run("streaming", host)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    got = dd.download()
+    for f in ("dp", "p", "dpu", "dpv", "pu", "pv"):
+        assert np.array_equal(want.a[f], got.a[f]), f
+    for f in ("temp", "saln", "th3d", "tracer", "u", "v"):
+        x = want.a[f]
+        assert np.max(np.abs(x - got.a[f])) / np.max(np.abs(x)) <= 1e-5, f
+    df = ol.diff_domains(host, got)
+    assert not df, f"GPU mixed differs from the host build of the same code: {df}"
