@@ -25,7 +25,7 @@ import hybgen_abi as abi  # noqa: E402
 
 METRIC = "column-layers remapped/s, FP64 vs mixed, GLBb tile; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
-PHASES = ["column", "halo", "uv", "ra", "halo2", "uv2"]
+PHASES = ["column", "halo", "uv", "ra", "halo2", "uv2", "regrid", "remap"]
 
 
 def bytes_per_column_phase_a(K, T):
@@ -33,6 +33,13 @@ def bytes_per_column_phase_a(K, T):
     write dp, T, S, th3d and T tracers (K levels each), read theta (K),
     write p (K+1), read depths and topiso, read ip (int32)."""
     return 8 * (2 * (4 + T) * K + K + (K + 1)) + 8 * 2 + 4
+
+
+def bytes_per_column_remap(K, T):
+    """Algorithmic HBM bytes of the p-column remap kernel per sea column:
+    read the new and the old interfaces (K+1 each), read and write T, S and
+    T tracers (K levels each), write th3d (K), read kp and the PCM mask."""
+    return 8 * (2 * (K + 1) + 2 * (2 + T) * K + K) + 4 + 8
 
 
 def bytes_per_column_call(K, T):
@@ -123,6 +130,75 @@ def mixed_leg(lib, dd, S, sea, args, stream):
             "error_window": "600x400 centre window of the same field"}
 
 
+def batch_leg(args):
+    """64 regional domains of random shape (idm 100-1200, jdm 100-900, kdm
+    22/32/41, shelf fraction 0-60%), FP64 PPM, stepped together by
+    hycom_hybgen_batch (one graph launch) after the autotuner has settled;
+    beside it the same domains stepped one hycom_hybgen_device call each."""
+    import numpy as np
+    import torch
+    lib = abi.load()
+    rng = np.random.default_rng(args.seed + 3)
+    doms, sea = [], []
+    for d in range(args.batch_domains):
+        idm, jdm = int(rng.integers(100, 1201)), int(rng.integers(100, 901))
+        kdm = int(rng.choice([22, 32, 41]))
+        shelf = float(rng.uniform(0.0, 0.6))
+        P, w = abi.workload_params("cfg1", idm=idm, jdm=jdm, kdm=kdm, shelf=shelf)
+        dd = abi.DeviceDomain(P, skip=("dpo",))
+        dd.synth(abi.make_synth(P, w, seed=args.seed + 17 * d))
+        doms.append(dd)
+        sea.append(sea_columns(dd) * kdm)
+    torch.cuda.synchronize()
+    layers = sum(sea)
+    print(f"[bench] batch of {len(doms)} domains, {layers} sea column-layers",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream().cuda_stream
+    # one call per domain
+    for dd in doms:
+        dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for dd in doms:
+            dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    t_single = (time.perf_counter() - t0) / args.steps
+    # the batch: autotune, then time
+    abi.check(lib.hycom_hybgen_batch_autotune(1))
+    batch = abi.Batch(doms)
+    calls = 0
+    while not batch.config()[2] and calls < 64:
+        batch.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        calls += 1
+    batch.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        batch.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    t_batch = (time.perf_counter() - t0) / args.steps
+    rc, gc, tuned = batch.config()
+    print(f"[bench] batch {1e3 * t_batch:.2f} ms/call, one call per domain "
+          f"{1e3 * t_single:.2f} ms; widths remap {rc} regrid {gc}", file=sys.stderr, flush=True)
+    lat.sort()
+    shapes = [f"{dd.P.idm}x{dd.P.jdm}x{dd.P.kdm}" for dd in doms]
+    del batch, doms
+    torch.cuda.empty_cache()
+    return {"value": layers / t_batch, "unit": "column-layers/s", "ms_per_call": 1e3 * t_batch,
+            "domains": len(shapes), "sea_column_layers": layers,
+            "one_call_per_domain": {"value": layers / t_single, "ms_per_step": 1e3 * t_single},
+            "speedup_vs_one_call_per_domain": t_single / t_batch,
+            "autotuned": {"remap_cols": rc, "regrid_cols": gc, "tuned": tuned,
+                          "tuning_calls": calls},
+            "latency_ms": {"p50": 1e3 * lat[len(lat) // 2], "p99": 1e3 * lat[-1]},
+            "shapes": shapes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +211,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (HYB_PREC_MIXED) leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched ragged domains leg")
+    ap.add_argument("--batch-domains", type=int, default=64)
     ap.add_argument("--kernel-path", type=int, default=0,
                     help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
     ap.add_argument("--remap-variant", type=int, default=None,
@@ -194,8 +273,15 @@ def main():
     total = t1 - t0
     ms_step = 1e3 * total / args.steps
     value = sea * P.kdm * args.steps / total
-    ms_a = ms[0] / max(cnt[0], 1)
-    bpc_a = bytes_per_column_phase_a(P.kdm, P.ntracr)
+    if cnt[7]:
+        # the streaming kernels: the p-column remap kernel on its own
+        kname = f"k_remap_s<{args.hybmap},3,0> (p-column remap, hybgenaj)"
+        ms_a = ms[7] / cnt[7]
+        bpc_a = bytes_per_column_remap(P.kdm, P.ntracr)
+    else:
+        kname = "k_phase_a (column regrid + remap, hybgenaj)"
+        ms_a = ms[0] / max(cnt[0], 1)
+        bpc_a = bytes_per_column_phase_a(P.kdm, P.ntracr)
     achieved = sea * bpc_a / (ms_a * 1e-3) / 1e9
     srt = sorted(step_s)
     p50 = srt[len(srt) // 2] * 1e3
@@ -204,12 +290,21 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
+    batch = None
+    if not args.no_batch:
+        del dd
+        torch.cuda.empty_cache()
+        batch = batch_leg(args)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_hybmap{args.hybmap}.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        if cnt[7]:
+            traffic = tj.get("bytes_per_launch_by_kernel", {}).get(kname.split(" ")[0])
+        else:
+            traffic = tj.get("bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -240,7 +335,8 @@ def main():
         "phase_ms": {PHASES[q]: ms[q] / cnt[q] for q in range(np_) if cnt[q]},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_phase_a (column regrid + remap, hybgenaj)",
+            "kernel": kname,
+            "kernel_ms": ms_a,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -253,6 +349,7 @@ def main():
         "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "batch": batch,
     }
     if mixed:
         mixed["speedup_vs_fp64"] = mixed["value"] / value

@@ -332,6 +332,295 @@ int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev) {
   return 0;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Batched ragged domains
+// ---------------------------------------------------------------------
+namespace {
+
+struct BatchPlan {
+  std::vector<unsigned char> key;  // the KParams and View of every domain
+  int ndom = 0;
+  KParams* dP = nullptr;
+  View* dV = nullptr;
+  Scratch* dW = nullptr;
+  int32_t* dI = nullptr;
+  size_t nI = 0;
+  BatchDevice B{};
+  BatchLaunch L{};
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // per time level n
+  // online autotuning
+  std::vector<std::pair<int, int>> cands;  // (remap cols, regrid cols)
+  std::vector<float> ms;  // best time of each candidate
+  int cand = 0;           // candidate measured by the pending events
+  int rep = 0;            // measurements of it so far
+  bool pending = false, tuned = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t done = nullptr;  // after the last launch of this plan's graphs
+  bool launched = false;
+  std::vector<KParams> hP;
+  std::vector<View> hV;
+};
+std::mutex g_batch_mu;
+BatchPlan g_plan;
+bool g_batch_autotune = true;
+
+void plan_drop_graphs(BatchPlan& b) {
+  for (int q = 0; q < 2; q++) {
+    if (b.exec[q]) (void)hipGraphExecDestroy(b.exec[q]);
+    if (b.graph[q]) (void)hipGraphDestroy(b.graph[q]);
+    b.exec[q] = nullptr;
+    b.graph[q] = nullptr;
+  }
+}
+
+// waits for the last launch of the plan's graphs before its arrays change
+int plan_quiesce(BatchPlan& b) {
+  if (b.launched) HIPCHK(hipEventSynchronize(b.done));
+  b.launched = false;
+  return 0;
+}
+
+void plan_free(BatchPlan& b) {
+  plan_drop_graphs(b);
+  if (b.dP) (void)hipFree(b.dP);
+  if (b.dV) (void)hipFree(b.dV);
+  if (b.dW) (void)hipFree(b.dW);
+  if (b.dI) (void)hipFree(b.dI);
+  b.dP = nullptr;
+  b.dV = nullptr;
+  b.dW = nullptr;
+  b.dI = nullptr;
+  b.nI = 0;
+  b.key.clear();
+}
+
+// block layouts of every launch for the current widths
+int plan_layout(BatchPlan& b) {
+  if (int rc = plan_quiesce(b)) return rc;
+  const int nd = b.ndom;
+  std::vector<int32_t> I;
+  auto group = [&](std::vector<int> doms, auto blocks_of, int key, BatchGroup* G) {
+    G->ndom = (int32_t)doms.size();
+    G->key = key;
+    const size_t od = I.size();
+    for (int d : doms) I.push_back(d);
+    const size_t op = I.size();
+    int64_t acc = 0;
+    I.push_back(0);
+    for (int d : doms) {
+      acc += blocks_of(b.hP[d]);
+      I.push_back((int32_t)acc);
+    }
+    G->nblocks = (int32_t)acc;
+    // offsets for now; turned into device pointers below
+    G->dom = (const int32_t*)(uintptr_t)od;
+    G->prefix = (const int32_t*)(uintptr_t)op;
+  };
+  auto rows = [](int cols) {
+    return [cols](const KParams& P) { return (int64_t)P.jj * ((P.ii + cols - 1) / cols); };
+  };
+  std::vector<int> all(nd);
+  for (int d = 0; d < nd; d++) all[d] = d;
+  BatchLaunch& L = b.L;
+  L.kmax = 0;
+  for (int d = 0; d < nd; d++) L.kmax = b.hP[d].kk > L.kmax ? b.hP[d].kk : L.kmax;
+  group(all, rows(L.regrid_cols), 0, &L.regrid);
+  L.nremap = 0;
+  for (int key = 0; key < 8; key++) {
+    std::vector<int> ds;
+    for (int d = 0; d < nd; d++)
+      if (b.hP[d].hybmap + 4 * b.hP[d].precision == key) ds.push_back(d);
+    if (!ds.empty()) group(ds, rows(L.remap_cols), key, &L.remap[L.nremap++]);
+  }
+  group(all, rows(64), 0, &L.rows64);
+  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.ii + 255) / 256); }, 0,
+        &L.halo_j);
+  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.jj + 2 + 255) / 256); }, 0,
+        &L.halo_i);
+  L.nuv = 0;
+  for (int key = 0; key < 8; key++) {
+    std::vector<int> ds;
+    for (int d = 0; d < nd; d++)
+      if (uv_method_of(b.hP[d]) + 4 * b.hP[d].precision == key) ds.push_back(d);
+    if (!ds.empty())
+      group(ds, [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); }, key,
+            &L.uv[L.nuv++]);
+  }
+  group(all, [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); }, 0,
+        &L.uvall);
+  if (I.size() > b.nI) {
+    if (b.dI) (void)hipFree(b.dI);
+    b.dI = nullptr;
+    HIPCHK(hipMalloc(&b.dI, sizeof(int32_t) * I.size()));
+    b.nI = I.size();
+  }
+  HIPCHK(hipMemcpy(b.dI, I.data(), sizeof(int32_t) * I.size(), hipMemcpyHostToDevice));
+  auto fix = [&](BatchGroup& G) {
+    G.dom = b.dI + (uintptr_t)G.dom;
+    G.prefix = b.dI + (uintptr_t)G.prefix;
+  };
+  fix(L.regrid);
+  for (int q = 0; q < L.nremap; q++) fix(L.remap[q]);
+  fix(L.rows64);
+  fix(L.halo_j);
+  fix(L.halo_i);
+  for (int q = 0; q < L.nuv; q++) fix(L.uv[q]);
+  fix(L.uvall);
+  plan_drop_graphs(b);
+  return 0;
+}
+
+int plan_graph(BatchPlan& b, int n) {
+  const int q = n - 1;
+  if (b.exec[q]) return 0;
+  if (!b.cap) HIPCHK(hipStreamCreateWithFlags(&b.cap, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_batch(b.B, b.L, n, b.cap);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b.cap, &g);
+  HIPCHK(e);
+  HIPCHK(e2);
+  b.graph[q] = g;
+  HIPCHK(hipGraphInstantiate(&b.exec[q], g, nullptr, nullptr, 0));
+  return 0;
+}
+
+int plan_build(BatchPlan& b, const hyb_domain* D, int ndom, std::vector<unsigned char>& key) {
+  if (int rc = plan_quiesce(b)) return rc;
+  plan_free(b);
+  b.ndom = ndom;
+  b.hP.resize(ndom);
+  b.hV.resize(ndom);
+  std::vector<Scratch> hW(ndom);
+  for (int d = 0; d < ndom; d++) {
+    b.hP[d] = kparams_of(D[d].P);
+    b.hV[d] = view_of(D[d].P, D[d].F);
+    int rc = workspace_for(D[d].F->p, b.hV[d].NIJ, D[d].P->kdm, &hW[d]);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMalloc(&b.dP, sizeof(KParams) * ndom));
+  HIPCHK(hipMalloc(&b.dV, sizeof(View) * ndom));
+  HIPCHK(hipMalloc(&b.dW, sizeof(Scratch) * ndom));
+  HIPCHK(hipMemcpy(b.dP, b.hP.data(), sizeof(KParams) * ndom, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.dV, b.hV.data(), sizeof(View) * ndom, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.dW, hW.data(), sizeof(Scratch) * ndom, hipMemcpyHostToDevice));
+  b.B = BatchDevice{b.dP, b.dV, b.dW};
+  int kmax = 0;
+  for (int d = 0; d < ndom; d++) kmax = b.hP[d].kk > kmax ? b.hP[d].kk : kmax;
+  // candidate widths; a remap workgroup holds (kmax+1) x cols doubles of LDS
+  b.cands.clear();
+  for (int rc : {64, 128, 256})
+    if (sizeof(double) * (kmax + 1) * rc <= 65536)
+      for (int gc : {256, 128, 64}) b.cands.push_back({rc, gc});
+  b.ms.assign(b.cands.size(), -1.0f);
+  b.cand = 0;
+  b.rep = 0;
+  b.pending = false;
+  b.tuned = !g_batch_autotune;
+  // first candidate (and the untuned widths): 64-column remaps, 256-column regrid
+  b.L.remap_cols = b.cands[0].first;
+  b.L.regrid_cols = b.cands[0].second;
+  if (!b.ev0) HIPCHK(hipEventCreate(&b.ev0));
+  if (!b.ev1) HIPCHK(hipEventCreate(&b.ev1));
+  if (!b.done) HIPCHK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+  b.key = key;
+  return plan_layout(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream) {
+  if (!D || ndom < 1) return fail(HYB_EINVAL, "empty batch");
+  if (m < 1 || m > 2 || n < 1 || n > 2) return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
+  int rc;
+  for (int d = 0; d < ndom; d++) {
+    if (!D[d].P || !D[d].F) return fail(HYB_ENULL, "domain %d: NULL params or fields", d);
+    if ((rc = hycom_hybgen_check_params(D[d].P))) return rc;
+    if ((rc = check_fields(D[d].P, D[d].F, 0))) return rc;
+    if (!stream_path_ok(kparams_of(D[d].P)))
+      return fail(HYB_EUNSUPPORTED, "domain %d: batches need nhybrd==kdm, no HYBGEN_DENFIX", d);
+    if (D[d].P->itracr801 || D[d].P->itracr802 || D[d].P->itracr803)
+      return fail(HYB_EUNSUPPORTED, "domain %d: no itracr diagnostics in batches", d);
+    for (int e = 0; e < d; e++)
+      if (D[e].F->p == D[d].F->p) return fail(HYB_EINVAL, "domains %d and %d share p", e, d);
+  }
+  if ((rc = device_ready())) return rc;
+  std::vector<unsigned char> key;
+  key.reserve(ndom * (sizeof(KParams) + sizeof(View)));
+  for (int d = 0; d < ndom; d++) {
+    KParams k = kparams_of(D[d].P);
+    View v = view_of(D[d].P, D[d].F);
+    const unsigned char* a = (const unsigned char*)&k;
+    key.insert(key.end(), a, a + sizeof k);
+    a = (const unsigned char*)&v;
+    key.insert(key.end(), a, a + sizeof v);
+  }
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  BatchPlan& b = g_plan;
+  if (b.key != key || b.ndom != ndom) {
+    if ((rc = plan_build(b, D, ndom, key))) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  // autotuning: read the pending measurement, move to the next candidate
+  if (b.pending) {
+    HIPCHK(hipEventSynchronize(b.ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, b.ev0, b.ev1));
+    b.ms[b.cand] = (b.rep == 0 || t < b.ms[b.cand]) ? t : b.ms[b.cand];
+    b.pending = false;
+    // two launches per candidate: the first one also loads the graph
+    if (++b.rep == 2) {
+      b.rep = 0;
+      int next = ++b.cand;
+      if (next >= (int)b.cands.size()) {
+        next = 0;
+        for (int q = 1; q < (int)b.cands.size(); q++)
+          if (b.ms[q] < b.ms[next]) next = q;
+        b.tuned = true;
+      }
+      b.L.remap_cols = b.cands[next].first;
+      b.L.regrid_cols = b.cands[next].second;
+      if ((rc = plan_layout(b))) return rc;
+    }
+  }
+  if ((rc = plan_graph(b, n))) return rc;
+  const bool measure = !b.tuned;
+  if (measure) HIPCHK(hipEventRecord(b.ev0, s));
+  HIPCHK(hipGraphLaunch(b.exec[n - 1], s));
+  if (measure) {
+    HIPCHK(hipEventRecord(b.ev1, s));
+    b.pending = true;
+  }
+  HIPCHK(hipEventRecord(b.done, s));
+  b.launched = true;
+  (void)m;
+  return 0;
+}
+
+int hycom_hybgen_batch_autotune(int enable) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  g_batch_autotune = enable != 0;
+  return 0;
+}
+
+int hycom_hybgen_batch_config(int32_t* remap_cols, int32_t* regrid_cols, int32_t* tuned) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  if (remap_cols) *remap_cols = g_plan.L.remap_cols;
+  if (regrid_cols) *regrid_cols = g_plan.L.regrid_cols;
+  if (tuned) *tuned = g_plan.tuned ? 1 : 0;
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag,
                         void* stream) {
   int rc = hycom_hybgen_check_params(P);

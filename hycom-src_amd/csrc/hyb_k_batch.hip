@@ -1,0 +1,209 @@
+// hyb_k_batch.hip -- batched ragged domains: every phase of hybgen over
+// all domains of a batch in one launch per kernel (one per remapper and
+// precision group for the remaps).  A launch's blocks are laid out domain
+// after domain; block b finds its domain by binary search in the group's
+// block prefix, then its row j and column slice from the domain's shape.
+// The domain's parameters, field view and workspace are read from device
+// arrays (uniform per block, so they come in through scalar loads).
+#include "hyb_launch.h"
+#include "hyb_stream.h"
+
+namespace hyb {
+
+// largest g with prefix[g] <= b (prefix[0] = 0, prefix[ng] = total)
+__device__ __forceinline__ int group_of(const int32_t* prefix, int ng, int b) {
+  int lo = 0, hi = ng;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// block -> (domain, j, first i) for a row kernel with `bw` threads per block
+struct RowPos {
+  int d, j, i0;
+};
+__device__ __forceinline__ RowPos row_pos(const BatchGroup& G, const KParams* Ps, int bw, int b) {
+  const int g = group_of(G.prefix, G.ndom, b);
+  const int d = G.dom[g];
+  const int local = b - G.prefix[g];
+  const int nbi = (Ps[d].ii + bw - 1) / bw;
+  return RowPos{d, local / nbi + 1, (local % nbi) * bw};
+}
+
+template <int BW>
+__global__ void __launch_bounds__(BW) kb_regrid(BatchDevice B, BatchGroup G, int n) {
+  const RowPos r = row_pos(G, B.P, BW, blockIdx.x);
+  const KParams& P = B.P[r.d];
+  const View& V = B.V[r.d];
+  const int i = r.i0 + threadIdx.x + 1;
+  if (i > P.ii) return;
+  if (V.ip[ij_of(V, i, r.j)] == 0) return;
+  regrid_stream(P, V, B.W[r.d], i, r.j, n);
+}
+
+// CW columns per workgroup; each lane's LDS column has stride CW doubles
+template <int METHOD, bool MIXED, int CW>
+__global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int n) {
+  extern __shared__ double lds[];
+  const RowPos r = row_pos(G, B.P, CW, blockIdx.x);
+  const KParams& P = B.P[r.d];
+  const View& V = B.V[r.d];
+  const Scratch& W = B.W[r.d];
+  const int i = r.i0 + threadIdx.x + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, r.j);
+  if (V.ip[c0] == 0) return;
+  if (W.kp[c0] & kUnsafe) return;
+  double* row = lds + threadIdx.x;
+  const double* pg = V.p + c0;
+  for (int k = 1; k <= P.kk + 1; k++) row[(k - 1) * CW] = pg[(int64_t)(k - 1) * V.NIJ];
+  LaneRow po{row, CW};
+  if (MIXED) remap_mixed<METHOD, 3>(P, V, W, i, r.j, n, po);
+  else remap_stream<METHOD, 3, 0>(P, V, W, i, r.j, n, po);
+}
+
+__global__ void __launch_bounds__(kWave) kb_remap_fb(BatchDevice B, BatchGroup G, int n) {
+  const RowPos r = row_pos(G, B.P, kWave, blockIdx.x);
+  const KParams& P = B.P[r.d];
+  const View& V = B.V[r.d];
+  const Scratch& W = B.W[r.d];
+  const int i = r.i0 + threadIdx.x + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, r.j);
+  if (V.ip[c0] == 0) return;
+  if (!(W.kp[c0] & kUnsafe)) return;
+  remap_fallback(P, V, W, i, r.j, n);
+}
+
+// p halo of every domain, levels 2..kk+1: the j rows, then the i columns
+// (mod_xc_sm.h:1374-1424).  Blocks: kk per domain and 256-wide slice.
+__global__ void __launch_bounds__(kBlock) kb_halo_j(BatchDevice B, BatchGroup G) {
+  const int g = group_of(G.prefix, G.ndom, blockIdx.x);
+  const int d = G.dom[g];
+  const KParams& P = B.P[d];
+  const int local = blockIdx.x - G.prefix[g];
+  const int nbi = (P.ii + kBlock - 1) / kBlock;
+  const int k = local / nbi + 2;
+  const int i = (local % nbi) * kBlock + threadIdx.x + 1;
+  if (i > P.ii) return;
+  p_halo_jrow(P, B.V[d], i, k);
+}
+__global__ void __launch_bounds__(kBlock) kb_halo_i(BatchDevice B, BatchGroup G) {
+  const int g = group_of(G.prefix, G.ndom, blockIdx.x);
+  const int d = G.dom[g];
+  const KParams& P = B.P[d];
+  const int local = blockIdx.x - G.prefix[g];
+  const int nbj = (P.jj + 2 + kBlock - 1) / kBlock;
+  const int k = local / nbj + 2;
+  const int j = (local % nbj) * kBlock + threadIdx.x;
+  if (j > P.jj + 1) return;
+  p_halo_icol(P, B.V[d], j, k);
+}
+
+// u- and v-points: the first half of a domain's blocks are u, then v
+template <int METHOD, bool MIXED>
+__global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int nl) {
+  extern __shared__ double lds[];
+  const int g = group_of(G.prefix, G.ndom, blockIdx.x);
+  const int d = G.dom[g];
+  const KParams& P = B.P[d];
+  const View& V = B.V[d];
+  const int nbi = (P.ii + kWave - 1) / kWave;
+  int local = blockIdx.x - G.prefix[g];
+  const bool isv = local >= P.jj * nbi;
+  if (isv) local -= P.jj * nbi;
+  const int j = local / nbi + 1;
+  const int i = (local % nbi) * kWave + threadIdx.x + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  const int32_t* msk = isv ? V.iv : V.iu;
+  if (msk[c0] == 0) return;
+  LaneRow dcol{lds + threadIdx.x, kWave};
+  const bool ok = uv_stream<METHOD, MIXED ? 0 : 2, MIXED>(P, V, i, j, nl, isv, dcol);
+  B.W[d].uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
+}
+
+__global__ void __launch_bounds__(kWave) kb_uv_fb(BatchDevice B, BatchGroup G, int nl) {
+  const int g = group_of(G.prefix, G.ndom, blockIdx.x);
+  const int d = G.dom[g];
+  const KParams& P = B.P[d];
+  const View& V = B.V[d];
+  const int nbi = (P.ii + kWave - 1) / kWave;
+  int local = blockIdx.x - G.prefix[g];
+  const bool isv = local >= P.jj * nbi;
+  if (isv) local -= P.jj * nbi;
+  const int j = local / nbi + 1;
+  const int i = (local % nbi) * kWave + threadIdx.x + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  const int32_t* msk = isv ? V.iv : V.iu;
+  if (msk[c0] == 0) return;
+  if (B.W[d].uvf[c0 + (isv ? V.NIJ : 0)] == 0) return;
+  point_c<64>(P, V, i, j, nl, isv);
+}
+
+template <int CW>
+static void remap_group_cw(const BatchDevice& B, const BatchGroup& G, int n, size_t lds,
+                           hipStream_t s) {
+  const dim3 grid(G.nblocks), blk(CW);
+  switch (G.key) {  // hybmap + 4 * precision
+    case 0: hipLaunchKernelGGL((kb_remap<0, false, CW>), grid, blk, lds, s, B, G, n); break;
+    case 1: hipLaunchKernelGGL((kb_remap<1, false, CW>), grid, blk, lds, s, B, G, n); break;
+    case 2: hipLaunchKernelGGL((kb_remap<2, false, CW>), grid, blk, lds, s, B, G, n); break;
+    case 3: hipLaunchKernelGGL((kb_remap<3, false, CW>), grid, blk, lds, s, B, G, n); break;
+    case 4: hipLaunchKernelGGL((kb_remap<0, true, CW>), grid, blk, lds, s, B, G, n); break;
+    case 5: hipLaunchKernelGGL((kb_remap<1, true, CW>), grid, blk, lds, s, B, G, n); break;
+    case 6: hipLaunchKernelGGL((kb_remap<2, true, CW>), grid, blk, lds, s, B, G, n); break;
+    default: hipLaunchKernelGGL((kb_remap<3, true, CW>), grid, blk, lds, s, B, G, n); break;
+  }
+}
+
+static void remap_group(const BatchDevice& B, const BatchGroup& G, int n, int kmax, int cw,
+                        hipStream_t s) {
+  const size_t lds = sizeof(double) * (kmax + 1) * cw;
+  if (cw == 256) remap_group_cw<256>(B, G, n, lds, s);
+  else if (cw == 128) remap_group_cw<128>(B, G, n, lds, s);
+  else remap_group_cw<64>(B, G, n, lds, s);
+}
+
+static void uv_group(const BatchDevice& B, const BatchGroup& G, int nl, size_t lds,
+                     hipStream_t s) {
+  const dim3 grid(G.nblocks), blk(kWave);
+  switch (G.key) {  // uv_method + 4 * precision
+    case 0: hipLaunchKernelGGL((kb_uv<0, false>), grid, blk, lds, s, B, G, nl); break;
+    case 1: hipLaunchKernelGGL((kb_uv<1, false>), grid, blk, lds, s, B, G, nl); break;
+    case 3: hipLaunchKernelGGL((kb_uv<3, false>), grid, blk, lds, s, B, G, nl); break;
+    case 4: hipLaunchKernelGGL((kb_uv<0, true>), grid, blk, lds, s, B, G, nl); break;
+    case 5: hipLaunchKernelGGL((kb_uv<1, true>), grid, blk, lds, s, B, G, nl); break;
+    default: hipLaunchKernelGGL((kb_uv<3, true>), grid, blk, lds, s, B, G, nl); break;
+  }
+}
+
+hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipStream_t s) {
+  if (L.regrid.nblocks > 0) {
+    const dim3 grid(L.regrid.nblocks);
+    if (L.regrid_cols == 256)
+      hipLaunchKernelGGL(kb_regrid<256>, grid, dim3(256), 0, s, B, L.regrid, n);
+    else if (L.regrid_cols == 128)
+      hipLaunchKernelGGL(kb_regrid<128>, grid, dim3(128), 0, s, B, L.regrid, n);
+    else
+      hipLaunchKernelGGL(kb_regrid<64>, grid, dim3(64), 0, s, B, L.regrid, n);
+  }
+  for (int q = 0; q < L.nremap; q++) remap_group(B, L.remap[q], n, L.kmax, L.remap_cols, s);
+  if (L.rows64.nblocks > 0)
+    hipLaunchKernelGGL(kb_remap_fb, dim3(L.rows64.nblocks), dim3(kWave), 0, s, B, L.rows64, n);
+  if (L.halo_j.nblocks > 0)
+    hipLaunchKernelGGL(kb_halo_j, dim3(L.halo_j.nblocks), dim3(kBlock), 0, s, B, L.halo_j);
+  if (L.halo_i.nblocks > 0)
+    hipLaunchKernelGGL(kb_halo_i, dim3(L.halo_i.nblocks), dim3(kBlock), 0, s, B, L.halo_i);
+  const size_t lds_u = sizeof(double) * L.kmax * kWave;
+  for (int q = 0; q < L.nuv; q++) uv_group(B, L.uv[q], n, lds_u, s);
+  if (L.uvall.nblocks > 0)
+    hipLaunchKernelGGL(kb_uv_fb, dim3(L.uvall.nblocks), dim3(kWave), 0, s, B, L.uvall, n);
+  return hipGetLastError();
+}
+
+}  // namespace hyb

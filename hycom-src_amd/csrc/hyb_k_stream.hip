@@ -52,6 +52,8 @@ static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, 
 
 bool stream_path_ok(const KParams& P) { return !P.ldenfix && P.nhybrd == P.kk; }
 
+int uv_method_of(const KParams& P) { return uv_method(P); }
+
 hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
                                 int n, int raflag, const StreamTuning& tune, hipStream_t s,
                                 PhaseTimer* T) {
@@ -62,12 +64,18 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   launch_diag_pre(P, V, n, s);
   {
     PhaseMark pm(T, s, 0);
-    hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
-    switch (P.hybmap) {
-      case 0: launch_remap_stream_m0(P, V, W, n, tune.remap_variant, s); break;
-      case 1: launch_remap_stream_m1(P, V, W, n, tune.remap_variant, s); break;
-      case 2: launch_remap_stream_m2(P, V, W, n, tune.remap_variant, s); break;
-      default: launch_remap_stream_m3(P, V, W, n, tune.remap_variant, s); break;
+    {
+      PhaseMark pk(T, s, 6);
+      hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
+    }
+    {
+      PhaseMark pk(T, s, 7);
+      switch (P.hybmap) {
+        case 0: launch_remap_stream_m0(P, V, W, n, tune.remap_variant, s); break;
+        case 1: launch_remap_stream_m1(P, V, W, n, tune.remap_variant, s); break;
+        case 2: launch_remap_stream_m2(P, V, W, n, tune.remap_variant, s); break;
+        default: launch_remap_stream_m3(P, V, W, n, tune.remap_variant, s); break;
+      }
     }
     launch_remap_fallback(P, V, W, n, s);
   }

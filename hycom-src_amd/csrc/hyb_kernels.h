@@ -12,7 +12,7 @@ namespace hyb {
 // Optional HIP events around each phase of one call, for measurement:
 // 0 column kernel (hybgenaj), 1 p halo, 2 u/v kernel (dpudpv+hybgenbj),
 // 3 Robert-Asselin kernel (hybgencj), 4 second halo, 5 second u/v kernel.
-constexpr int kPhases = 6;
+constexpr int kPhases = 8;
 struct PhaseTimer {
   hipEvent_t start[kPhases], stop[kPhases];
   bool used[kPhases];
@@ -32,6 +32,8 @@ hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W,
 // True when the streaming kernels cover this parameter pack (any kdm,
 // nhybrd==kdm, no HYBGEN_DENFIX).
 bool stream_path_ok(const KParams& P);
+// remapper of the u/v phase: 0 PCM, 1 PLM, 3 PPM (hybgen.F90:1960-1975)
+int uv_method_of(const KParams& P);
 // Kernel variants of the streaming family (measurement and tuning):
 // remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
 // uv_prefetch the prefetch depth of the u/v walk (0 or 2).
@@ -44,6 +46,39 @@ struct StreamTuning {
 hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
                                 int n, int raflag, const StreamTuning& tune, hipStream_t s,
                                 PhaseTimer* T = nullptr);
+
+// ---------------------------------------------------------------------
+// Batched ragged domains (hyb_k_batch.hip).  Device arrays of the batch's
+// domains, and the block layout of each launch: a group lists domains
+// (dom[0..ndom)) and their block prefix (prefix[0..ndom], prefix[0] = 0,
+// prefix[ndom] = nblocks); key selects the kernel template of the group.
+// ---------------------------------------------------------------------
+struct BatchDevice {
+  const KParams* P;
+  const View* V;
+  const Scratch* W;
+};
+struct BatchGroup {
+  const int32_t* dom;
+  const int32_t* prefix;
+  int32_t ndom, nblocks, key;
+};
+constexpr int kMaxGroups = 8;
+struct BatchLaunch {
+  int32_t kmax;        // largest kdm of the batch (LDS size)
+  int32_t remap_cols;  // columns per remap workgroup: 64, 128 or 256
+  int32_t regrid_cols; // columns per regrid workgroup: 64, 128 or 256
+  BatchGroup regrid;                 // row blocks of regrid_cols, all domains
+  BatchGroup remap[kMaxGroups];      // row blocks of remap_cols, by hybmap + 4*precision
+  int32_t nremap;
+  BatchGroup rows64;                 // 64-wide row blocks, all domains
+  BatchGroup halo_j, halo_i;         // kk x 256-wide slices
+  BatchGroup uv[kMaxGroups];         // u+v 64-wide blocks, by uv method + 4*precision
+  int32_t nuv;
+  BatchGroup uvall;                  // u+v 64-wide blocks, all domains
+};
+// Enqueues one hybgen(m,n,.false.) step of every domain of the batch.
+hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipStream_t s);
 
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,

@@ -21,6 +21,8 @@ PREC_MIXED = 1
 HYB_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EUNSUPPORTED", -3: "EDEVICE", -4: "ENOMEM",
           -5: "ENODEVICE", -6: "ENULL", -7: "ENONFINITE", -8: "EBUSY"}
+for _code, _name in ERRORS.items():
+    globals()["HYB_" + _name] = _code
 
 _INT_FIELDS = ["idm", "jdm", "ii", "jj", "nbdy", "kdm", "ntracr", "nhybrd",
                "nsigma", "hybmap", "hybflg", "nreg", "isopcm", "mxlmy",
@@ -79,11 +81,20 @@ SIGNATURES = {
     "hycom_hybgen_timing": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_timing_read": (ctypes.c_int, [_P(ctypes.c_double), _P(ctypes.c_int32),
                                                 ctypes.c_int]),
+    "hycom_hybgen_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_void_p]),
+    "hycom_hybgen_batch_autotune": (ctypes.c_int, [ctypes.c_int]),
+    "hycom_hybgen_batch_config": (ctypes.c_int, [_P(ctypes.c_int32), _P(ctypes.c_int32),
+                                                 _P(ctypes.c_int32)]),
     "hycom_hybgen_synth_host": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_synth_device": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
 }
+
+class HybDomain(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_void_p), ("F", ctypes.c_void_p)]
+
 
 _lib = None
 
@@ -306,3 +317,27 @@ class DeviceDomain:
         check(load().hycom_hybgen_device(ctypes.byref(self.P), ctypes.byref(F), m, n,
                                          raflag, stream))
         return self
+
+
+class Batch:
+    """A set of DeviceDomains stepped together by hycom_hybgen_batch."""
+
+    def __init__(self, domains):
+        self.domains = list(domains)
+        self._F = [d.fields() for d in self.domains]
+        self._D = (HybDomain * len(self.domains))()
+        for q, (d, F) in enumerate(zip(self.domains, self._F)):
+            self._D[q].P = ctypes.addressof(d.P)
+            self._D[q].F = ctypes.addressof(F)
+
+    def hybgen(self, m=1, n=2, stream=None):
+        check(load().hycom_hybgen_batch(ctypes.addressof(self._D), len(self.domains), m, n,
+                                        stream))
+        return self
+
+    @staticmethod
+    def config():
+        """(remap columns, regrid columns, tuned) of the last batch."""
+        a, b, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(load().hycom_hybgen_batch_config(ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, bool(t.value)

@@ -191,7 +191,9 @@ int hycom_hybgen_tune(int knob, int value);
  * enabled, every hycom_hybgen_device() call records HIP events around its
  * phases on its stream: 0 column kernel (hybgenaj), 1 p halo, 2 u/v kernel
  * (dpudpv + hybgenbj), 3 Robert-Asselin kernel (hybgencj), 4 second halo,
- * 5 second u/v kernel.  _read waits for the recorded calls, returns the
+ * 5 second u/v kernel; within phase 0 of the streaming kernels, 6 the
+ * regrid kernel alone and 7 the p-column remap kernel alone (without its
+ * fallback columns).  _read waits for the recorded calls, returns the
  * summed milliseconds and call counts per phase, and forgets them. */
 int hycom_hybgen_timing(int enable);
 int hycom_hybgen_timing_read(double* ms_sum, int32_t* count, int nphase);
@@ -218,6 +220,38 @@ typedef struct hyb_synth {
   int32_t extreme;         /* 1: 40 C / 40 psu fronts and 1e-3 m layers     */
   int32_t reserved;
 } hyb_synth;
+
+/* ---------------------------------------------------------------------- */
+/* Batched ragged domains (no reference counterpart: the reference runs    */
+/* one domain per process; this serves ensembles and regional sets)       */
+/* ---------------------------------------------------------------------- */
+
+/* One domain of a batch: its parameters and its device field pointers. */
+typedef struct hyb_domain {
+  const hyb_params* P;
+  const hyb_fields* F;
+} hyb_domain;
+
+/* One hybgen(m,n,.false.) step of each of ndom independent domains, of any
+ * shapes and kdm (<= 64), FP64 or mixed per domain, enqueued on `stream`.
+ * Every phase is one launch over all domains (one per remapper and
+ * precision for the remaps), captured once into a HIP graph per batch and
+ * replayed; the same domain set on the next call replays the same graph.
+ * Each domain's result is bitwise the one of hycom_hybgen_device on that
+ * domain alone.  Requires nhybrd == kdm, no HYBGEN_DENFIX, no itracr
+ * diagnostics (HYB_EUNSUPPORTED otherwise). */
+int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream);
+
+/* Online autotuning of the batch kernels' workgroup widths (columns per
+ * workgroup of the regrid and of the remap, which sets the remap's LDS
+ * per workgroup): while enabled (the default), the first calls of a new
+ * domain set each try one candidate, timed with HIP events, and the
+ * fastest is kept for that set.  Results do not depend on the widths. */
+int hycom_hybgen_batch_autotune(int enable);
+
+/* Widths in use for the last batch: columns per workgroup of the remap
+ * and of the regrid; *tuned is 1 once the search has finished. */
+int hycom_hybgen_batch_config(int32_t* remap_cols, int32_t* regrid_cols, int32_t* tuned);
 
 /* Fills every field of F (host pointers) and theta/masks/depths for the
  * window described by S and P, in the layout above, time level n, and

@@ -101,3 +101,47 @@ def test_device_entry_fails_without_gpu():
     rc = lib.hycom_hybgen(ctypes.byref(P), ctypes.byref(F), 1, 2, 0)
     assert rc == -5  # HYB_ENODEVICE
     assert (d.a["temp"] == before).all()  # nothing computed on the CPU
+
+
+def _host_batch(specs):
+    """hyb_domain array over host-side fields (only validated on the CPU);
+    the caller keeps the returned domains alive while D is used."""
+    doms = []
+    for over in specs:
+        P = _params(idm=16, jdm=12, **over)
+        doms.append(abi.HostDomain(P))
+    Fs = [d.fields() for d in doms]
+    D = (abi.HybDomain * len(doms))()
+    for q, (d, F) in enumerate(zip(doms, Fs)):
+        D[q].P = ctypes.addressof(d.P)
+        D[q].F = ctypes.addressof(F)
+    return D, doms, Fs
+
+
+def test_batch_rejects_bad_batches():
+    lib = abi.load()
+    D, keep, _ = _host_batch([{}])
+    assert lib.hycom_hybgen_batch(ctypes.addressof(D), 0, 1, 2, None) == abi.HYB_EINVAL
+    assert lib.hycom_hybgen_batch(None, 1, 1, 2, None) == abi.HYB_EINVAL
+    assert lib.hycom_hybgen_batch(ctypes.addressof(D), 1, 3, 2, None) == abi.HYB_EINVAL
+    # nhybrd < kdm has no streaming kernels: refused, not run another way
+    D, keep, _ = _host_batch([{}, dict(nhybrd=30)])
+    assert _rc(keep[1].P) == 0
+    rc = lib.hycom_hybgen_batch(ctypes.addressof(D), 2, 1, 2, None)
+    assert rc == abi.HYB_EUNSUPPORTED
+    assert b"domain 1" in lib.hycom_hybgen_last_error()
+    # two domains over the same p array
+    D, doms, Fs = _host_batch([{}, {}])
+    D[1].F = ctypes.addressof(Fs[0])
+    assert lib.hycom_hybgen_batch(ctypes.addressof(D), 2, 1, 2, None) == abi.HYB_EINVAL
+
+
+def test_batch_fails_without_gpu():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is visible")
+    except Exception:
+        pass
+    D, keep, _ = _host_batch([{}, dict(kdm=22)])
+    assert abi.load().hycom_hybgen_batch(ctypes.addressof(D), 2, 1, 2, None) == abi.HYB_ENODEVICE
