@@ -199,6 +199,83 @@ def batch_leg(args):
             "shapes": shapes}
 
 
+def serve_leg(args):
+    """Request loop: Poisson arrivals (mean args.rate per s) for
+    args.serve_seconds, each request nsteps ~ U{1..10} hybgen steps of one
+    of args.serve_domains resident windows of the GLBb field (idm 150-600,
+    jdm 100-400, kdm 41, 4 tracers, PPM), FP64 or mixed with equal odds.
+    Latency is completion minus submission on the server's clock."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed + 5)
+    w0 = abi.WORKLOADS["glbb"]
+    srv = abi.Server(max_active=args.serve_slots)
+    ids, shapes = [], []
+    for d in range(args.serve_domains):
+        idm, jdm = int(rng.integers(150, 601)), int(rng.integers(100, 401))
+        i0 = int(rng.integers(0, w0["idm"] - idm))
+        j0 = int(rng.integers(0, w0["jdm"] - jdm))
+        P, w = abi.workload_params("glbb", idm=idm, jdm=jdm, hybmap=args.hybmap)
+        S = abi.make_synth(P, w, seed=args.seed, itdm=w0["idm"], jtdm=w0["jdm"], i0=i0, j0=j0)
+        ids.append(srv.add_domain(P, S))
+        shapes.append(f"{idm}x{jdm}@{i0},{j0}")
+    # every domain made resident by a one-step request before the timed run
+    for d in ids:
+        srv.submit(d, 1, abi.PREC_FP64, tag=-1)
+    warm = []
+    while len(warm) < len(ids):
+        warm += srv.poll(wait_ms=100)
+    assert all(c.status == 0 for c in warm), "warm-up request failed"
+    st0 = srv.stats()
+    # arrivals
+    n_req = int(rng.poisson(args.rate * args.serve_seconds))
+    gaps = rng.exponential(1.0 / args.rate, size=n_req)
+    t_arr = np.cumsum(gaps)
+    doms = rng.integers(0, len(ids), size=n_req)
+    steps = rng.integers(1, 11, size=n_req)
+    prec = rng.integers(0, 2, size=n_req)
+    print(f"[bench] serving {n_req} requests over {t_arr[-1]:.1f} s "
+          f"({len(ids)} resident domains)", file=sys.stderr, flush=True)
+    done = []
+    t0 = time.perf_counter()
+    q = 0
+    last_print = t0
+    while q < n_req or len(done) < n_req:
+        now = time.perf_counter() - t0
+        while q < n_req and t_arr[q] <= now:
+            srv.submit(ids[int(doms[q])], int(steps[q]), int(prec[q]), tag=q)
+            q += 1
+        wait = 1 if q >= n_req else max(0, int(1e3 * (t_arr[q] - now)))
+        done += srv.poll(wait_ms=min(wait, 2))
+        if time.perf_counter() - last_print > 30:
+            last_print = time.perf_counter()
+            print(f"[bench] serving: {q} submitted, {len(done)} done", file=sys.stderr, flush=True)
+        if now > 10 * args.serve_seconds + 60:
+            raise RuntimeError(f"serving run stalled: {len(done)} of {n_req} done")
+    st = srv.stats()
+    srv.close()
+    bad = [c for c in done if c.status != 0]
+    assert not bad, f"{len(bad)} requests failed"
+    lat = np.sort(np.array([c.t_done - c.t_submit for c in done]))
+    qwait = np.array([c.t_start - c.t_submit for c in done])
+    span = max(c.t_done for c in done) - min(c.t_submit for c in done)
+    layers = sum(c.column_layers for c in done)
+    iters = st.iterations - st0.iterations
+    return {"value": layers / span, "unit": "column-layers/s",
+            "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
+                           "p99": 1e3 * float(np.percentile(lat, 99)),
+                           "max": 1e3 * float(lat[-1])},
+            "queue_wait_ms_p50": 1e3 * float(np.median(qwait)),
+            "requests": n_req, "rate_per_s": args.rate, "span_s": span,
+            "mixed_fraction": float(prec.mean()),
+            "iterations": iters,
+            "mean_active_per_iteration": (st.active_sum - st0.active_sum) / max(iters, 1),
+            "max_active": st.max_active_seen, "slots": args.serve_slots,
+            "resident_domains": st.resident_domains,
+            "resident_GB": st.resident_bytes / 1e9,
+            "workload": (f"{len(ids)} windows of the GLBb 4500x3298x41 field, idm 150-600, "
+                         f"jdm 100-400, 4 tracers, PPM; nsteps U{{1..10}}; FP64 or mixed 50/50")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +291,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched ragged domains leg")
     ap.add_argument("--batch-domains", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop leg")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-domains", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--kernel-path", type=int, default=0,
                     help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
     ap.add_argument("--remap-variant", type=int, default=None,
@@ -290,11 +372,14 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
-    batch = None
-    if not args.no_batch:
-        del dd
+    batch = serve = None
+    if not (args.no_batch and args.no_serve):
+        del dd  # the GLBb fields make room for the other legs
         torch.cuda.empty_cache()
+    if not args.no_batch:
         batch = batch_leg(args)
+    if not args.no_serve:
+        serve = serve_leg(args)
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_hybmap{args.hybmap}.json")
@@ -350,6 +435,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "batch": batch,
+        "serve": serve,
     }
     if mixed:
         mixed["speedup_vs_fp64"] = mixed["value"] / value

@@ -12,12 +12,13 @@
 #include <vector>
 
 #include "../../include/hycom_hybgen.h"
+#include "hyb_host.h"
 #include "hyb_kernels.h"
 #include "hyb_params_conv.h"
 
 using namespace hyb;
 
-namespace {
+namespace hyb {
 
 thread_local char g_err[512] = "";
 
@@ -29,13 +30,7 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                           \
-  do {                                                                         \
-    hipError_t e_ = (expr);                                                    \
-    if (e_ != hipSuccess)                                                      \
-      return fail(e_ == hipErrorOutOfMemory ? HYB_ENOMEM : HYB_EDEVICE,        \
-                  "%s: %s", #expr, hipGetErrorString(e_));                     \
-  } while (0)
+const char* last_error() { return g_err; }
 
 struct Shape {
   int64_t NI = 0, NJ = 0, NIJ = 0;
@@ -89,6 +84,84 @@ int check_fields(const hyb_params* P, const hyb_fields* F, int raflag) {
   if (raflag && !F->dpo) return fail(HYB_ENULL, "field dpo is NULL (hybgen_raflag set)");
   return 0;
 }
+
+int batch_domain_ok(const hyb_params* P, const hyb_fields* F, int d) {
+  if (!P || !F) return fail(HYB_ENULL, "domain %d: NULL params or fields", d);
+  int rc;
+  if ((rc = hycom_hybgen_check_params(P))) return rc;
+  if ((rc = check_fields(P, F, 0))) return rc;
+  if (!stream_path_ok(kparams_of(P)))
+    return fail(HYB_EUNSUPPORTED, "domain %d: batches need nhybrd==kdm, no HYBGEN_DENFIX", d);
+  if (P->itracr801 || P->itracr802 || P->itracr803)
+    return fail(HYB_EUNSUPPORTED, "domain %d: no itracr diagnostics in batches", d);
+  return 0;
+}
+
+void batch_layout(const KParams* hP, int nd, BatchLaunch& L, std::vector<int32_t>& I) {
+  auto group = [&](const std::vector<int>& doms, auto blocks_of, int key, BatchGroup* G) {
+    G->ndom = (int32_t)doms.size();
+    G->key = key;
+    const size_t od = I.size();
+    for (int d : doms) I.push_back(d);
+    const size_t op = I.size();
+    int64_t acc = 0;
+    I.push_back(0);
+    for (int d : doms) {
+      acc += blocks_of(hP[d]);
+      I.push_back((int32_t)acc);
+    }
+    G->nblocks = (int32_t)acc;
+    G->dom = (const int32_t*)(uintptr_t)od;
+    G->prefix = (const int32_t*)(uintptr_t)op;
+  };
+  auto rows = [](int cols) {
+    return [cols](const KParams& P) { return (int64_t)P.jj * ((P.ii + cols - 1) / cols); };
+  };
+  auto uvblocks = [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); };
+  std::vector<int> all(nd);
+  for (int d = 0; d < nd; d++) all[d] = d;
+  L.kmax = 0;
+  for (int d = 0; d < nd; d++) L.kmax = hP[d].kk > L.kmax ? hP[d].kk : L.kmax;
+  group(all, rows(L.regrid_cols), 0, &L.regrid);
+  L.nremap = 0;
+  for (int key = 0; key < kMaxGroups; key++) {
+    std::vector<int> ds;
+    for (int d = 0; d < nd; d++)
+      if (hP[d].hybmap + 4 * hP[d].precision == key) ds.push_back(d);
+    if (!ds.empty()) group(ds, rows(L.remap_cols), key, &L.remap[L.nremap++]);
+  }
+  group(all, rows(64), 0, &L.rows64);
+  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.ii + 255) / 256); }, 0,
+        &L.halo_j);
+  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.jj + 2 + 255) / 256); }, 0,
+        &L.halo_i);
+  L.nuv = 0;
+  for (int key = 0; key < kMaxGroups; key++) {
+    std::vector<int> ds;
+    for (int d = 0; d < nd; d++)
+      if (uv_method_of(hP[d]) + 4 * hP[d].precision == key) ds.push_back(d);
+    if (!ds.empty()) group(ds, uvblocks, key, &L.uv[L.nuv++]);
+  }
+  group(all, uvblocks, 0, &L.uvall);
+}
+
+void batch_bind(BatchLaunch& L, const int32_t* dI) {
+  auto fix = [&](BatchGroup& G) {
+    G.dom = dI + (uintptr_t)G.dom;
+    G.prefix = dI + (uintptr_t)G.prefix;
+  };
+  fix(L.regrid);
+  for (int q = 0; q < L.nremap; q++) fix(L.remap[q]);
+  fix(L.rows64);
+  fix(L.halo_j);
+  fix(L.halo_i);
+  for (int q = 0; q < L.nuv; q++) fix(L.uv[q]);
+  fix(L.uvall);
+}
+
+}  // namespace hyb
+
+namespace {
 
 // device mirrors for the copy-mode call
 struct Mirror {
@@ -401,57 +474,8 @@ void plan_free(BatchPlan& b) {
 // block layouts of every launch for the current widths
 int plan_layout(BatchPlan& b) {
   if (int rc = plan_quiesce(b)) return rc;
-  const int nd = b.ndom;
   std::vector<int32_t> I;
-  auto group = [&](std::vector<int> doms, auto blocks_of, int key, BatchGroup* G) {
-    G->ndom = (int32_t)doms.size();
-    G->key = key;
-    const size_t od = I.size();
-    for (int d : doms) I.push_back(d);
-    const size_t op = I.size();
-    int64_t acc = 0;
-    I.push_back(0);
-    for (int d : doms) {
-      acc += blocks_of(b.hP[d]);
-      I.push_back((int32_t)acc);
-    }
-    G->nblocks = (int32_t)acc;
-    // offsets for now; turned into device pointers below
-    G->dom = (const int32_t*)(uintptr_t)od;
-    G->prefix = (const int32_t*)(uintptr_t)op;
-  };
-  auto rows = [](int cols) {
-    return [cols](const KParams& P) { return (int64_t)P.jj * ((P.ii + cols - 1) / cols); };
-  };
-  std::vector<int> all(nd);
-  for (int d = 0; d < nd; d++) all[d] = d;
-  BatchLaunch& L = b.L;
-  L.kmax = 0;
-  for (int d = 0; d < nd; d++) L.kmax = b.hP[d].kk > L.kmax ? b.hP[d].kk : L.kmax;
-  group(all, rows(L.regrid_cols), 0, &L.regrid);
-  L.nremap = 0;
-  for (int key = 0; key < 8; key++) {
-    std::vector<int> ds;
-    for (int d = 0; d < nd; d++)
-      if (b.hP[d].hybmap + 4 * b.hP[d].precision == key) ds.push_back(d);
-    if (!ds.empty()) group(ds, rows(L.remap_cols), key, &L.remap[L.nremap++]);
-  }
-  group(all, rows(64), 0, &L.rows64);
-  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.ii + 255) / 256); }, 0,
-        &L.halo_j);
-  group(all, [](const KParams& P) { return (int64_t)P.kk * ((P.jj + 2 + 255) / 256); }, 0,
-        &L.halo_i);
-  L.nuv = 0;
-  for (int key = 0; key < 8; key++) {
-    std::vector<int> ds;
-    for (int d = 0; d < nd; d++)
-      if (uv_method_of(b.hP[d]) + 4 * b.hP[d].precision == key) ds.push_back(d);
-    if (!ds.empty())
-      group(ds, [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); }, key,
-            &L.uv[L.nuv++]);
-  }
-  group(all, [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); }, 0,
-        &L.uvall);
+  batch_layout(b.hP.data(), b.ndom, b.L, I);
   if (I.size() > b.nI) {
     if (b.dI) (void)hipFree(b.dI);
     b.dI = nullptr;
@@ -459,17 +483,7 @@ int plan_layout(BatchPlan& b) {
     b.nI = I.size();
   }
   HIPCHK(hipMemcpy(b.dI, I.data(), sizeof(int32_t) * I.size(), hipMemcpyHostToDevice));
-  auto fix = [&](BatchGroup& G) {
-    G.dom = b.dI + (uintptr_t)G.dom;
-    G.prefix = b.dI + (uintptr_t)G.prefix;
-  };
-  fix(L.regrid);
-  for (int q = 0; q < L.nremap; q++) fix(L.remap[q]);
-  fix(L.rows64);
-  fix(L.halo_j);
-  fix(L.halo_i);
-  for (int q = 0; q < L.nuv; q++) fix(L.uv[q]);
-  fix(L.uvall);
+  batch_bind(b.L, b.dI);
   plan_drop_graphs(b);
   return 0;
 }
@@ -540,13 +554,7 @@ int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream
   if (m < 1 || m > 2 || n < 1 || n > 2) return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
   int rc;
   for (int d = 0; d < ndom; d++) {
-    if (!D[d].P || !D[d].F) return fail(HYB_ENULL, "domain %d: NULL params or fields", d);
-    if ((rc = hycom_hybgen_check_params(D[d].P))) return rc;
-    if ((rc = check_fields(D[d].P, D[d].F, 0))) return rc;
-    if (!stream_path_ok(kparams_of(D[d].P)))
-      return fail(HYB_EUNSUPPORTED, "domain %d: batches need nhybrd==kdm, no HYBGEN_DENFIX", d);
-    if (D[d].P->itracr801 || D[d].P->itracr802 || D[d].P->itracr803)
-      return fail(HYB_EUNSUPPORTED, "domain %d: no itracr diagnostics in batches", d);
+    if ((rc = batch_domain_ok(D[d].P, D[d].F, d))) return rc;
     for (int e = 0; e < d; e++)
       if (D[e].F->p == D[d].F->p) return fail(HYB_EINVAL, "domains %d and %d share p", e, d);
   }

@@ -62,6 +62,31 @@ class HybSynth(ctypes.Structure):
                 ("extreme", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class HybDomain(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_void_p), ("F", ctypes.c_void_p)]
+
+
+class HybServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("max_active", ctypes.c_int32),
+                ("remap_cols", ctypes.c_int32), ("regrid_cols", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class HybCompletion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_int64), ("domain", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("nsteps", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("column_layers", ctypes.c_int64)]
+
+
+class HybServerStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("column_layers", ctypes.c_int64), ("resident_bytes", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64), ("opens", ctypes.c_int64),
+                ("evictions", ctypes.c_int64), ("resident_domains", ctypes.c_int32),
+                ("max_active_seen", ctypes.c_int32), ("active_sum", ctypes.c_double)]
+
+
 # every symbol include/hycom_hybgen.h declares, with its ctypes signature
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -86,15 +111,25 @@ SIGNATURES = {
     "hycom_hybgen_batch_autotune": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_batch_config": (ctypes.c_int, [_P(ctypes.c_int32), _P(ctypes.c_int32),
                                                  _P(ctypes.c_int32)]),
+    "hycom_hybgen_server_create": (ctypes.c_int, [_P(HybServerConfig), _P(ctypes.c_void_p)]),
+    "hycom_hybgen_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "hycom_hybgen_server_add_domain": (ctypes.c_int, [ctypes.c_void_p, _P(HybParams),
+                                                      _P(HybSynth), _P(HybFields),
+                                                      _P(ctypes.c_int32)]),
+    "hycom_hybgen_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                  ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.c_int64]),
+    "hycom_hybgen_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(HybCompletion),
+                                                ctypes.c_int32, ctypes.c_int32,
+                                                _P(ctypes.c_int32)]),
+    "hycom_hybgen_server_read_domain": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                       _P(HybFields)]),
+    "hycom_hybgen_server_stats": (ctypes.c_int, [ctypes.c_void_p, _P(HybServerStats)]),
     "hycom_hybgen_synth_host": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_synth_device": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
 }
-
-class HybDomain(ctypes.Structure):
-    _fields_ = [("P", ctypes.c_void_p), ("F", ctypes.c_void_p)]
-
 
 _lib = None
 
@@ -341,3 +376,57 @@ class Batch:
         a, b, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(load().hycom_hybgen_batch_config(ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         return a.value, b.value, bool(t.value)
+
+
+class Server:
+    """hycom_hybgen_server_*: resident domains stepped on request, with
+    continuous batching of the active requests."""
+
+    def __init__(self, hbm_budget=0, max_active=0, remap_cols=0, regrid_cols=0):
+        C = HybServerConfig(hbm_budget, max_active, remap_cols, regrid_cols, 0)
+        h = ctypes.c_void_p()
+        check(load().hycom_hybgen_server_create(ctypes.byref(C), ctypes.byref(h)))
+        self.h = h
+        self.params = {}
+
+    def add_domain(self, P, S=None, host=None):
+        """Registers a domain from a synthetic description or a HostDomain."""
+        did = ctypes.c_int32()
+        F = host.fields() if host is not None else None
+        check(load().hycom_hybgen_server_add_domain(
+            self.h, ctypes.byref(P), ctypes.byref(S) if S is not None else None,
+            ctypes.byref(F) if F is not None else None, ctypes.byref(did)))
+        self.params[did.value] = P
+        return did.value
+
+    def submit(self, dom, nsteps, precision=PREC_FP64, tag=0):
+        check(load().hycom_hybgen_server_submit(self.h, dom, nsteps, precision, tag))
+
+    def poll(self, max_n=256, wait_ms=0):
+        buf = (HybCompletion * max_n)()
+        n = ctypes.c_int32()
+        check(load().hycom_hybgen_server_poll(self.h, buf, max_n, wait_ms, ctypes.byref(n)))
+        return [buf[q] for q in range(n.value)]
+
+    def read_domain(self, dom):
+        d = HostDomain(self.params[dom])
+        F = d.fields()
+        F.dpo = None
+        check(load().hycom_hybgen_server_read_domain(self.h, dom, ctypes.byref(F)))
+        return d
+
+    def stats(self):
+        st = HybServerStats()
+        check(load().hycom_hybgen_server_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            check(load().hycom_hybgen_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

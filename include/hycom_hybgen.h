@@ -253,6 +253,72 @@ int hycom_hybgen_batch_autotune(int enable);
  * and of the regrid; *tuned is 1 once the search has finished. */
 int hycom_hybgen_batch_config(int32_t* remap_cols, int32_t* regrid_cols, int32_t* tuned);
 
+/* ---------------------------------------------------------------------- */
+/* Request loop over resident domains (no reference counterpart: serving) */
+/* ---------------------------------------------------------------------- */
+
+/* A server keeps many domains' state resident on the current device and
+ * steps them on request.  A request asks for nsteps hybgen(1,2,.false.)
+ * calls of one domain, FP64 or mixed.  A scheduler thread runs
+ * iterations: each iteration steps every active request once, all
+ * domains together in the batched kernels; requests that arrive while
+ * others run are admitted at the next iteration as slots and device
+ * memory free up (continuous batching), and a request leaves the active
+ * set after its last step.  Requests of one domain run in submission
+ * order.  A domain's state is created on the device (synthetic generator
+ * or a copy of host arrays) when a request for it is first admitted;
+ * when the memory budget is short, idle domains are evicted to host
+ * memory, least recently used first, and come back on their next request.
+ * Each result is bitwise the one of hycom_hybgen_device run alone. */
+typedef struct hyb_server hyb_server;
+
+typedef struct hyb_server_config {
+  int64_t hbm_budget;   /* device bytes for resident domains; 0: 90% of free */
+  int32_t max_active;   /* requests stepped per iteration, 1..256; 0: 64    */
+  int32_t remap_cols;   /* columns per remap workgroup 64|128|256; 0: 64    */
+  int32_t regrid_cols;  /* columns per regrid workgroup 64|128|256; 0: 256  */
+  int32_t reserved;
+} hyb_server_config;
+
+typedef struct hyb_completion {
+  int64_t tag;            /* the caller's tag from submit                    */
+  int32_t domain;
+  int32_t status;         /* 0, or the HYB_E* code the request failed with  */
+  int32_t nsteps, precision;
+  double t_submit;        /* seconds since the server was created           */
+  double t_start;         /* admitted into an iteration                      */
+  double t_done;          /* last iteration seen complete on the device      */
+  int64_t column_layers;  /* nsteps x sea columns x kdm                      */
+} hyb_completion;
+
+typedef struct hyb_server_stats {
+  int64_t iterations, completed, column_layers;
+  int64_t resident_bytes, budget_bytes;
+  int64_t opens, evictions;
+  int32_t resident_domains, max_active_seen;
+  double active_sum;      /* sum over iterations of the active requests     */
+} hyb_server_stats;
+
+int hycom_hybgen_server_create(const hyb_server_config* C, hyb_server** out);
+/* Cancels queued requests (status HYB_EBUSY), waits for the device, frees. */
+int hycom_hybgen_server_destroy(hyb_server* srv);
+/* Registers a domain.  Its initial state is the synthetic generator S (both
+ * time levels, m=1 n=2) or a copy of the host arrays F taken now; the
+ * device copy is made at first admission.  P->precision is ignored (each
+ * request names its own).  Same limits as hycom_hybgen_batch. */
+int hycom_hybgen_server_add_domain(hyb_server* srv, const hyb_params* P, const hyb_synth* S,
+                                   const hyb_fields* F, int32_t* domain_id);
+int hycom_hybgen_server_submit(hyb_server* srv, int32_t domain_id, int32_t nsteps,
+                               int32_t precision, int64_t tag);
+/* Moves up to max completions into out (*n of them), waiting up to wait_ms
+ * for the first one.  Returns the server's error code once it has failed. */
+int hycom_hybgen_server_poll(hyb_server* srv, hyb_completion* out, int32_t max,
+                             int32_t wait_ms, int32_t* n);
+/* Copies a domain's current state into host arrays (NULL fields skipped);
+ * HYB_EBUSY while requests of the domain are queued or running. */
+int hycom_hybgen_server_read_domain(hyb_server* srv, int32_t domain_id, const hyb_fields* F);
+int hycom_hybgen_server_stats(hyb_server* srv, hyb_server_stats* st);
+
 /* Fills every field of F (host pointers) and theta/masks/depths for the
  * window described by S and P, in the layout above, time level n, and
  * copies level n to level m (m != n).  Parameters in P (dp0k, thbase...)

@@ -145,3 +145,26 @@ def test_batch_fails_without_gpu():
         pass
     D, keep, _ = _host_batch([{}, dict(kdm=22)])
     assert abi.load().hycom_hybgen_batch(ctypes.addressof(D), 2, 1, 2, None) == abi.HYB_ENODEVICE
+
+
+def test_server_config_checks_and_no_gpu():
+    lib = abi.load()
+    h = ctypes.c_void_p()
+    for bad in (dict(max_active=-1), dict(max_active=257), dict(remap_cols=96),
+                dict(regrid_cols=32), dict(hbm_budget=-5)):
+        C = abi.HybServerConfig(**bad)
+        assert lib.hycom_hybgen_server_create(ctypes.byref(C), ctypes.byref(h)) == abi.HYB_EINVAL
+        assert not h.value
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return
+    except Exception:
+        pass
+    assert lib.hycom_hybgen_server_create(None, ctypes.byref(h)) == abi.HYB_ENODEVICE
+    assert not h.value
+    # NULL handles are refused, not dereferenced
+    n = ctypes.c_int32()
+    assert lib.hycom_hybgen_server_submit(None, 0, 1, 0, 0) == abi.HYB_ENULL
+    assert lib.hycom_hybgen_server_poll(None, None, 0, 0, ctypes.byref(n)) == abi.HYB_ENULL
+    assert lib.hycom_hybgen_server_destroy(None) == 0
