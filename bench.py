@@ -302,6 +302,8 @@ def main():
                     help="streaming remap variant 0..5 (hycom_hybgen_tune knob 0)")
     ap.add_argument("--uv-prefetch", type=int, default=None,
                     help="streaming u/v prefetch depth 0 or 2 (knob 1)")
+    ap.add_argument("--uv-recompute", type=int, default=None,
+                    help="u/v new thicknesses from LDS (0) or formed again from p (1; knob 2)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -317,6 +319,8 @@ def main():
         abi.check(lib.hycom_hybgen_tune(0, args.remap_variant))
     if args.uv_prefetch is not None:
         abi.check(lib.hycom_hybgen_tune(1, args.uv_prefetch))
+    if args.uv_recompute is not None:
+        abi.check(lib.hycom_hybgen_tune(2, args.uv_recompute))
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     P, w = abi.workload_params(args.workload, hybmap=args.hybmap)

@@ -686,6 +686,11 @@ int hycom_hybgen_tune(int knob, int value) {
     g_tune.uv_prefetch = value;
     return 0;
   }
+  if (knob == 2) {
+    if (value != 0 && value != 1) return fail(HYB_EINVAL, "u/v recompute %d: 0 or 1", value);
+    g_tune.uv_recompute = value;
+    return 0;
+  }
   return fail(HYB_EINVAL, "tuning knob %d", knob);
 }
 

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch 
   regrid_stream(P, V, W, i, j, n);
 }
 
-template <int METHOD, int PD, bool MIXED = false>
+template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>
 __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, int nl) {
   extern __shared__ double lds[];
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -24,28 +24,36 @@ __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, in
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
   LaneRow dcol{lds + threadIdx.x, kWave};
-  const bool ok = uv_stream<METHOD, PD, MIXED>(P, V, i, j, nl, isv, dcol);
+  const bool ok = uv_stream<METHOD, PD, MIXED, RECOMP>(P, V, i, j, nl, isv, dcol);
   W.uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
 }
 
-static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, int nl,
-                             int pd, hipStream_t s) {
+template <int M, bool RECOMP>
+static void uv_stream_m(const KParams& P, const View& V, const Scratch& W, int nl, int pd,
+                        hipStream_t s) {
   dim3 w1(kWave);
   dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
-  const size_t lds = sizeof(double) * P.kk * kWave;
+  const size_t lds = RECOMP ? 0 : sizeof(double) * P.kk * kWave;
+  if (P.precision == 1)
+    hipLaunchKernelGGL((k_uv_s<M, 0, true, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
+  else if (pd > 0)
+    hipLaunchKernelGGL((k_uv_s<M, 2, false, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
+  else
+    hipLaunchKernelGGL((k_uv_s<M, 0, false, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
+}
+
+static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, int nl,
+                             const StreamTuning& tune, hipStream_t s) {
   const int m = uv_method(P);
-  if (P.precision == 1) {
-    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 0, true>), gw3, w1, lds, s, P, V, W, nl);
-    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 0, true>), gw3, w1, lds, s, P, V, W, nl);
-    else hipLaunchKernelGGL((k_uv_s<3, 0, true>), gw3, w1, lds, s, P, V, W, nl);
-  } else if (pd > 0) {
-    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 2>), gw3, w1, lds, s, P, V, W, nl);
-    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 2>), gw3, w1, lds, s, P, V, W, nl);
-    else hipLaunchKernelGGL((k_uv_s<3, 2>), gw3, w1, lds, s, P, V, W, nl);
+  const int pd = tune.uv_prefetch;
+  if (tune.uv_recompute) {
+    if (m == 0) uv_stream_m<0, true>(P, V, W, nl, pd, s);
+    else if (m == 1) uv_stream_m<1, true>(P, V, W, nl, pd, s);
+    else uv_stream_m<3, true>(P, V, W, nl, pd, s);
   } else {
-    if (m == 0) hipLaunchKernelGGL((k_uv_s<0, 0>), gw3, w1, lds, s, P, V, W, nl);
-    else if (m == 1) hipLaunchKernelGGL((k_uv_s<1, 0>), gw3, w1, lds, s, P, V, W, nl);
-    else hipLaunchKernelGGL((k_uv_s<3, 0>), gw3, w1, lds, s, P, V, W, nl);
+    if (m == 0) uv_stream_m<0, false>(P, V, W, nl, pd, s);
+    else if (m == 1) uv_stream_m<1, false>(P, V, W, nl, pd, s);
+    else uv_stream_m<3, false>(P, V, W, nl, pd, s);
   }
   launch_uv_fallback(P, V, W, nl, s);
 }
@@ -88,7 +96,7 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   if (e != hipSuccess) return e;
   {
     PhaseMark pm(T, s, 2);
-    launch_uv_stream(P, V, W, n, tune.uv_prefetch, s);
+    launch_uv_stream(P, V, W, n, tune, s);
   }
   if (raflag) {
     {
@@ -101,7 +109,7 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
     }
     if (e != hipSuccess) return e;
     PhaseMark pm(T, s, 5);
-    launch_uv_stream(P, V, W, m, tune.uv_prefetch, s);
+    launch_uv_stream(P, V, W, m, tune, s);
   }
   return hipGetLastError();
 }

@@ -649,6 +649,36 @@ struct PoSum {
   }
 };
 
+// The same prsf(k) without the LDS column: the new thicknesses are formed
+// again from p at the two neighbouring p-points as the walk asks for them
+// (the operations of the first pass of uv_stream, so the same bits).
+struct PoRecompute {
+  const double* p0;
+  const double* p1;
+  int64_t S;
+  double depthu;
+  double prv = 0.0, cur = 0.0, b = 0.0;  // prsf(k-1), prsf(k), min(depthu, p-mean at k)
+  int k = 1;
+  HYB_HD double operator()(int kq) {
+    if (kq == 1) {
+      prv = 0.0;
+      cur = 0.0;
+      b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
+      k = 1;
+      return cur;
+    }
+    while (k < kq) {
+      const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
+      const double d = dmax(0.0, a - b);
+      b = a;
+      prv = cur;
+      cur = cur + d;
+      k++;
+    }
+    return cur;
+  }
+};
+
 // Output sinks.  emit(kc, v, certain, thick): v[f] is output kc of field
 // f before the clamp; certain means it lies inside the range of the input
 // levels read so far, so the clamp against the whole column cannot change
@@ -691,12 +721,13 @@ struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
     }
   }
 };
+template <class PO>
 struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
   double* uu;
   int64_t S;
   double ub;    // ubavg or vbavg at time level nl
   double ptop;  // prsf(kk+1)
-  const PoSum* po;
+  const PO* po;
   uint64_t fix = 0;
   HYB_HD void emit(int kc, const double* v, bool certain, bool thick) {
     if (thick || po->prv <= ptop - kOnemm) {
@@ -1585,7 +1616,7 @@ HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int 
 // would overwrite a level not yet read (the test of regrid_stream), it
 // writes pu only and returns false; point_c then redoes the point.
 // ---------------------------------------------------------------------
-template <int METHOD, int PD, bool MIXED = false>
+template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>
 HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bool isv,
                       const LaneRow& dcol) {
   const int K = P.kk;
@@ -1610,7 +1641,7 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
     const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
     const double d = dmax(0.0, a - b);
     b = a;
-    dcol(k) = d;
+    if (!RECOMP) dcol(k) = d;
     pr = pr + d;
     pu[(int64_t)k * S] = pr;
     // output k ends in old layer <= k-3 only if pres(k-2) >= prsf(k+1)
@@ -1626,13 +1657,27 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   const double zx = q2;
   if (have && cand < zx) return false;
   PiSum pi{dpu, S};
-  PoSum po{dcol};
-  UvSink sink{uu, S, ub, pr, &po};
   double* fp[1] = {uu};
-  if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
-  else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+  if (RECOMP) {
+    PoRecompute po{p0, p1, S, depthu};
+    UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
+    if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
 #pragma unroll 2
-  for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
+    for (int k = 1; k <= K; k++) {
+      const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
+      dpu[(int64_t)(k - 1) * S] = dmax(0.0, a - bb);
+      bb = a;
+    }
+  } else {
+    PoSum po{dcol};
+    UvSink<PoSum> sink{uu, S, ub, pr, &po};
+    if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+#pragma unroll 2
+    for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
+  }
   return true;
 }
 

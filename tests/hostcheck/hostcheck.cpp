@@ -162,6 +162,7 @@ extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, i
 #include "hyb_stream.h"
 
 static int64_t g_unsafe_cols = 0;
+static int g_uv_recompute = 0;  // 1: u/v walks without the LDS column
 static int64_t g_uv_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
@@ -252,6 +253,12 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
               case 1: ok = uv_stream<1, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
               default: ok = uv_stream<3, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
             }
+          } else if (g_uv_recompute) {
+            switch (uv_method(P)) {
+              case 0: ok = uv_stream<0, 2, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 1: ok = uv_stream<1, 0, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              default: ok = uv_stream<3, 2, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
+            }
           } else switch (uv_method(P)) {
             case 0: ok = uv_stream<0, 2>(P, V, i, j, nl, isv != 0, dcol); break;
             case 1: ok = uv_stream<1, 0>(P, V, i, j, nl, isv != 0, dcol); break;
@@ -275,6 +282,15 @@ extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F,
   g_uv_fallback = 0;
   run_stream(K, V, m, n, raflag);
   return 0;
+}
+
+// the streaming family with the LDS-free u/v variant
+extern "C" int hostcheck_hybgen_stream_rc(const hyb_params* P, const hyb_fields* F, int m,
+                                          int n, int raflag) {
+  g_uv_recompute = 1;
+  const int rc = hostcheck_hybgen_stream(P, F, m, n, raflag);
+  g_uv_recompute = 0;
+  return rc;
 }
 
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }

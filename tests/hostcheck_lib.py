@@ -10,7 +10,8 @@ import hybgen_abi as abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SO = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
 FAMILIES = {"lane-arrays": "hostcheck_hybgen", "register-resident": "hostcheck_hybgen_fast",
-            "streaming": "hostcheck_hybgen_stream"}
+            "streaming": "hostcheck_hybgen_stream",
+            "streaming-recompute": "hostcheck_hybgen_stream_rc"}
 _lib = None
 
 
@@ -32,7 +33,7 @@ def covers(family, P):
     ok = P.ldenfix == 0 and P.nhybrd == P.kdm
     if family == "register-resident":
         return ok and P.kdm in (22, 32, 41)
-    if family == "streaming":
+    if family in ("streaming", "streaming-recompute"):
         return ok
     return True
 

@@ -176,3 +176,25 @@ def test_mixed_precision_on_gpu(gpu, hybmap):
         assert np.max(np.abs(x - got.a[f])) / np.max(np.abs(x)) <= 1e-5, f
     df = ol.diff_domains(host, got)
     assert not df, f"GPU mixed differs from the host build of the same code: {df}"
+
+
+@pytest.mark.parametrize("kdm,hybmap,extreme", [(41, 2, 0), (22, 1, 1), (64, 0, 1), (17, 3, 1)])
+def test_streaming_uv_without_lds(gpu, kdm, hybmap, extreme):
+    """The u/v walk that forms its new thicknesses again from p instead of
+    keeping them in LDS (tuning knob 2) gives the oracle's bits."""
+    dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
+    P, w = _case("uv-recompute", idm=90, jdm=57, kdm=kdm, hybmap=hybmap, nsigma=min(8, kdm),
+                 hybiso=3.0 if hybmap == 1 else 0.01, **dx)
+    _select(3, P)
+    lib = abi.load()
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm + 5, extreme=extreme))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    abi.check(lib.hycom_hybgen_tune(2, 1))
+    try:
+        dd.hybgen(m=1, n=2)
+        got = dd.download()
+    finally:
+        abi.check(lib.hycom_hybgen_tune(2, 0))
+        abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")

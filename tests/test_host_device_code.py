@@ -28,6 +28,7 @@ def _draw(rng, kdms):
 
 @pytest.mark.parametrize("family,kdms", [
     ("streaming", [4, 5, 9, 17, 22, 29, 32, 41, 50, 63, 64]),
+    ("streaming-recompute", [5, 17, 22, 41, 64]),
     ("register-resident", [22, 32, 41]),
     ("lane-arrays", [5, 22, 41, 64]),
 ])
