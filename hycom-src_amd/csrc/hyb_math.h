@@ -66,6 +66,34 @@ constexpr double c111 = c011 - c024 * (kRpdb * kRpdb);
 HYB_HD double dmax(double a, double b) { return a > b ? a : b; }
 HYB_HD double dmin(double a, double b) { return a < b ? a : b; }
 
+// Reciprocals of the mixed mode (HYB_PREC_MIXED; no reference
+// counterpart).  On the device: the hardware reciprocal (v_rcp_f32 /
+// v_rcp_f64, about 1 ulp), the double one refined by one Newton step; on
+// the host the IEEE quotient.  The two agree to a few ulp, not bitwise.
+// Arguments are normal and positive wherever they are used.
+HYB_HD float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+HYB_HD double frcp64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+// Quotients of the reconstruction in arithmetic type R: double keeps the
+// reference's IEEE division, float (the mixed mode) multiplies by frcp.
+HYB_HD double grcp(double x) { return 1.0 / x; }
+HYB_HD float grcp(float x) { return frcp(x); }
+HYB_HD double gdiv(double a, double b) { return a / b; }
+HYB_HD float gdiv(float a, float b) { return a * frcp(b); }
+
 // sig(t,s) = sig_n*(1/sig_d) - 1000, stmt_fns.h:505-511
 HYB_HD double sig(double t, double s) {
   double sn = c101 + t * (c002 + t * (c103 + t * c004)) +

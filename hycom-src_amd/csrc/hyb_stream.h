@@ -32,6 +32,8 @@
 #pragma once
 
 #include "hyb_column.h"
+#include <type_traits>
+
 #include "hyb_fast.h"
 
 namespace hyb {
@@ -786,7 +788,7 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
     // hybgen_plm_coefs (2393-2458) on dprs
     const bool flat = (l == 1 || l == K || lc_l || dprs_0 <= thin);
     R qcen = R(0);
-    if (!flat) qcen = dprs_0 / (dprs_0 + R(0.5) * (dprs_m + dprs_p));
+    if (!flat) qcen = gdiv(dprs_0, dprs_0 + R(0.5) * (dprs_m + dprs_p));
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       c1l[f] = R(0);
@@ -810,16 +812,16 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
     if (jn <= K - 1) {
       const R dpjp_l = dpi_0 + dpi_p;
       dpjp_n = dpi_p + dpi_p2;
-      qdpjp_n = R(1) / dpjp_n;  // qdpjp(l+1)
+      qdpjp_n = grcp(dpjp_n);  // qdpjp(l+1)
       dp2jp_l = dpi_0 + dpjp_l;
       dpj2p_n = dpjp_n + dpi_p2;
-      dpq3 = dpi_p / (dpi_0 + dpjp_n);
+      dpq3 = gdiv(dpi_p, dpi_0 + dpjp_n);
       if (do_al) {
         dpjp_lm = dpi_m + dpi_0;
         const R dpj2p_l = dpjp_l + dpi_p;
-        qdp2jp_l = R(1) / dp2jp_l;
-        qdpj2p_l = R(1) / dpj2p_l;
-        qdp4 = R(1) / (dpjp_lm + dpjp_n);
+        qdp2jp_l = grcp(dp2jp_l);
+        qdpj2p_l = grcp(dpj2p_l);
+        qdp4 = grcp(dpjp_lm + dpjp_n);
       }
     }
     const bool lim = l >= 2 && l <= K - 1;
@@ -889,9 +891,9 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
     const int jn = l + 1;
     const bool first_full = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
     R wq_n = R(0), qmjp = R(0), dpjm2jp = R(0);
-    if (jn <= K - 1) wq_n = R(1) / (dpi_p + dpi_p2);  // qdpjm(l+2)
+    if (jn <= K - 1) wq_n = grcp(dpi_p + dpi_p2);  // qdpjm(l+2)
     if (first_full) {
-      qmjp = R(1) / (dpi_0 + dpi_p + dpi_p2);
+      qmjp = grcp(dpi_0 + dpi_p + dpi_p2);
       dpjm2jp = dpi_0 + R(2) * dpi_p + dpi_p2;
     }
     const bool second = l >= 2 && l <= K - 1 && !(lc_l || dpi_0 <= thin);
@@ -928,7 +930,7 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
         if (jn <= K) {
           const R q002 = gmax(st.wz2[f], dsmll);
           const R q001 = gmax(nz1, dsmll);
-          zw3n = (q001 * st.wc2[f] + q002 * n1) / (q001 + q002);
+          zw3n = gdiv(q001 * st.wc2[f] + q002 * n1, q001 + q002);
         }
         c1l[f] = st.wc1[f];
         c2l[f] = st.wc2[f];
@@ -970,10 +972,11 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
 // fill); zx = pi(kk+1).  PD levels of every field (and of the old
 // interfaces) are loaded ahead of the window, to cover memory latency.
 // ---------------------------------------------------------------------
-template <int METHOD, int NF, int PD, class PI, class PO, class SINK>
+template <int METHOD, int NF, int PD, bool FAST = false, class PI, class PO, class SINK>
 HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
                       int nf, int kp, uint64_t lcm, int64_t S) {
   constexpr int PQ = PD > 0 ? PD : 1;
+  using CR = typename std::conditional<FAST, float, double>::type;
   const double thin = kDpthin;
   const double dsmll = 1.0e-8;
   auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
@@ -982,7 +985,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1), s(l), s(l+1), s(l+2)
   double smin[NF], smax[NF], skp[NF], sfirst[NF];
   double acc[NF], ov[NF], prev[NF];
-  CoefState<double, NF> cst;                   // reconstruction state
+  CoefState<CR, NF> cst;                       // reconstruction state
   double sq[NF][PQ];                           // s(l+3..l+2+PD), loaded ahead
   double prq[PQ];                              // raw(l+3..l+2+PD) of pi
   int xl = 4;                                  // next level to load
@@ -1019,10 +1022,10 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
       sfirst[f] = s1[f];
 #pragma unroll
       for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? load(f, 4 + q) : 0.0;
-      if (METHOD == 2) cst.alc[f] = s1[f];  // al(1) = s(1)
-      if (METHOD == 3) {                    // first sweep of layer 1
-        cst.wc1[f] = s1[f];
-        cst.wc2[f] = s1[f];
+      if (METHOD == 2) cst.alc[f] = (CR)s1[f];  // al(1) = s(1)
+      if (METHOD == 3) {                        // first sweep of layer 1
+        cst.wc1[f] = (CR)s1[f];
+        cst.wc2[f] = (CR)s1[f];
       }
     }
   }
@@ -1043,9 +1046,9 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   cst.qdpjp_l = 0.0;
   cst.wq_l = 0.0;
   if (K >= 2) {
-    const double d1 = dmax(pv2 - pv1, thin), d2 = dmax(pv3 - pv2, thin);
-    cst.qdpjp_l = 1.0 / (d1 + d2);
-    cst.wq_l = 1.0 / (d1 + d2);
+    const CR d1 = gmax((CR)(pv2 - pv1), (CR)thin), d2 = gmax((CR)(pv3 - pv2), (CR)thin);
+    cst.qdpjp_l = grcp(d1 + d2);
+    cst.wq_l = grcp(d1 + d2);
   }
 
   // walk state (2729-2735)
@@ -1088,11 +1091,50 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
     const bool lc_l = lcmb(l);
     // ------------------------------------------ coefficients of layer l
     double c1l[NF], c2l[NF], c3l[NF];
-    coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, dprs_m, dprs_0, dprs_p,
-                                   dpi_m, dpi_0, dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l, c2l,
-                                   c3l);
+    if constexpr (!FAST) {
+      coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, dprs_m, dprs_0, dprs_p,
+                                     dpi_m, dpi_0, dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l,
+                                     c2l, c3l);
+    } else {
+      // mixed mode: the reconstruction in float; the polynomial handed to
+      // the FP64 piece integrals keeps the layer mean s(l) (PPM: c3 from
+      // s(l) in FP64), so the remap still conserves to FP64 roundoff
+      const float thf = (float)thin;
+      const float fm = (float)dprs_m, f0 = (float)dprs_0, fq = (float)dprs_p,
+                  fq2 = (float)dprs_p2;
+      float sf0[NF], sf1[NF], sf2[NF], sf3[NF], g1[NF], g2[NF], g3[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        sf0[f] = (float)s0[f];
+        sf1[f] = (float)s1[f];
+        sf2[f] = (float)s2[f];
+        sf3[f] = (float)s3[f];
+      }
+      coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lc_l, thf, 1.0e-8f, fm, f0, fq,
+                                    gmax(fm, thf), gmax(f0, thf), gmax(fq, thf), gmax(fq2, thf),
+                                    sf0, sf1, sf2, sf3, cst, g1, g2, g3);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        c1l[f] = g1[f];
+        c2l[f] = g2[f];
+        c3l[f] = g3[f];
+        if (METHOD == 2) {
+          if (g2[f] == 0.0f && g3[f] == 0.0f) {
+            if (g1[f] == sf1[f]) c1l[f] = s1[f];  // flat: exactly s(l)
+          } else {
+            c3l[f] = 6.0 * s1[f] - 3.0 * (c1l[f] + (c1l[f] + c2l[f]));
+          }
+        }
+      }
+    }
 
     const double pil = pv1, pil1 = pv2, dprsl = dprs_0;
+    // x of a depth in layer l; the mixed mode multiplies by a reciprocal
+    const double rdl = FAST ? frcp64(dmax(dprsl, thin)) : 0.0;
+    auto xof = [&](double z) -> double {
+      return FAST ? (z - pil) * rdl : (z - pil) / dmax(dprsl, thin);
+    };
+    auto third = [](double x) -> double { return FAST ? x * (1.0 / 3.0) : x / 3.0; };
     // Every output still to come ends in this layer when it reaches the
     // bottom, and the walk stops here: fold the levels not read yet into
     // the column range now, before any of them is overwritten.
@@ -1112,7 +1154,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
 #pragma unroll
           for (int f = 0; f < NF; f++) v[f] = (kc != 1) ? prev[f] : sfirst[f];
         } else {
-          const double xb = (zb_act - pil) / dmax(dprsl, thin);
+          const double xb = xof(zb_act);
           const double dpb = zb_act - pil;
           double qb0 = 0.0, qb1 = 0.0, qb2 = 0.0;
           if (METHOD == 1) {
@@ -1121,13 +1163,14 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
           } else if (METHOD == 2) {
             qb0 = xb;
             qb1 = xb * xb * 0.5;
-            qb2 = xb * xb * xb / 3.0;
+            qb2 = third(xb * xb * xb);
           } else if (METHOD == 3) {
             qb1 = (xb - 1.0) * (xb - 1.0);
             qb2 = qb1 - 1.0 + xb;
             qb0 = 1.0 - qb1 - qb2;
           }
           const double dz = zb_act - zt_act;
+          const double rdz = FAST ? frcp64(dz) : 0.0;
 #pragma unroll
           for (int f = 0; f < NF; f++) {
             v[f] = 0.0;
@@ -1144,7 +1187,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
               } else {
                 a = a + dpb * (qb0 * (s1[f] - o) + qb1 * (c1l[f] - o) + qb2 * (c2l[f] - o));
               }
-              v[f] = o + a / dz;
+              v[f] = FAST ? o + a * rdz : o + a / dz;
             }
           }
         }
@@ -1174,16 +1217,17 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
 #pragma unroll
             for (int f = 0; f < NF; f++) v[f] = s1[f];
           } else {
-            const double xt = (zt - pil) / dmax(dprsl, thin);
-            const double xb = (zb - pil) / dmax(dprsl, thin);
+            const double xt = xof(zt);
+            const double xb = xof(zb);
             const double dz = zb - zt;
+            const double rdz = FAST ? frcp64(dz) : 0.0;
             double qt0 = 0.0, qt1 = 0.0, qt2 = 0.0;
             if (METHOD == 1) {
               qt1 = (xb * xb - xt * xt - (xb - xt)) * 0.5;
             } else if (METHOD == 2) {
               qt0 = (xb - xt);
               qt1 = (xb * xb - xt * xt) * 0.5;
-              qt2 = (xb * xb * xb - xt * xt * xt) / 3.0;
+              qt2 = third(xb * xb * xb - xt * xt * xt);
             } else {
               qt1 = xb * xb + xt * xt + xb * xt + 1.0 - 2.0 * (xb + xt);
               qt2 = qt1 - 1.0 + (xb + xt);
@@ -1194,12 +1238,12 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
               if (f < nf) {
                 if (METHOD == 1) {
                   const double sz = dprsl * (c1l[f] * qt1);
-                  v[f] = s1[f] + sz / dz;
+                  v[f] = FAST ? s1[f] + sz * rdz : s1[f] + sz / dz;
                 } else if (METHOD == 2) {
                   const double oo = s1[f];
                   const double sz =
                       dprsl * ((c1l[f] - oo) * qt0 + (c2l[f] + c3l[f]) * qt1 - c3l[f] * qt2);
-                  v[f] = oo + sz / dz;
+                  v[f] = FAST ? oo + sz * rdz : oo + sz / dz;
                 } else {
                   const double oo = s1[f];
                   const double sz = qt1 * (c1l[f] - oo) + qt2 * (c2l[f] - oo);
@@ -1235,7 +1279,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
           accon = true;
           if (!thn) {
             const int mid = (l + lb) / 2;
-            const double xt = (zt - pil) / dmax(dprsl, thin);
+            const double xt = xof(zt);
             const double dpt = pil1 - zt;
             double qt0 = 0.0, qt1 = 0.0, qt2 = 0.0;
             if (METHOD == 1) {
@@ -1244,7 +1288,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
             } else if (METHOD == 2) {
               qt0 = (1.0 - xt);
               qt1 = (1.0 - xt * xt) * 0.5;
-              qt2 = (1.0 - xt * xt * xt) / 3.0;
+              qt2 = third(1.0 - xt * xt * xt);
             } else if (METHOD == 3) {
               qt1 = xt * (xt - 1.0);
               qt2 = qt1 + xt;
@@ -1314,237 +1358,17 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
 }
 
 // ---------------------------------------------------------------------
-// Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart).  The
-// walk is the same, but:
-//  * the reconstruction coefficients and their metrics are float
-//    (coef_layer<float>);
-//  * each old layer l is cut by the new interfaces into pieces; a piece
-//    [za,zb] carries s(l)*(zb-za) in FP64 plus (zb-za) times the average
-//    of the reconstruction's deviation from s(l) over the piece, evaluated
-//    in float in factored form, so thin pieces lose no precision;
-//  * the float rounding of a layer (its deviations sum to zero exactly)
-//    is returned to the layer's pieces in proportion to their widths, so
-//    every old layer hands out exactly s(l)*dp(l) in FP64 and the column
-//    integral is conserved to FP64 roundoff (absent thin outputs and
-//    clamping, as in the reference);
-//  * an output is its FP64 piece sum divided by its thickness (FP64).
+// Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
+// walk of walk_core with FAST set.  The reconstruction (coef_layer) runs
+// in float with hardware reciprocals; the piece integrals, the output sums
+// and the interfaces stay FP64, and the quotients of the walk multiply by
+// refined FP64 reciprocals.  Every old layer's polynomial keeps its mean
+// s(l) exactly (the PPM c3 is formed from s(l) in FP64; PLM and WENO keep
+// it by construction), so each layer hands out s(l)*dp(l) up to FP64
+// roundoff and the column integrals are conserved as in the FP64 walk.
 // The overturn/unmix, the interfaces and every decision of the regrid stay
 // FP64 and bitwise the reference's.
 // ---------------------------------------------------------------------
-template <int METHOD, int NF>
-__attribute__((always_inline)) HYB_HD void avgdev_f(const float* c1, const float* c2,
-                                                    const float* c3, const float* sf, float xa,
-                                                    float xe, float* out) {
-  // average over [xa,xe] of r(x) - s for the reconstruction r of one layer
-  const float sum = xa + xe;
-  const float q2 = xa * xa + xa * xe + xe * xe;  // 3 * average of x^2
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    if (METHOD == 0) {
-      out[f] = 0.0f;
-    } else if (METHOD == 1) {  // r = s + c1 (x - 1/2)
-      out[f] = c1[f] * (0.5f * sum - 0.5f);
-    } else if (METHOD == 2) {  // r = c1 + (c2 + c3) x - c3 x^2
-      out[f] = (c1[f] - sf[f]) + (c2[f] + c3[f]) * (0.5f * sum) - c3[f] * (q2 * (1.0f / 3.0f));
-    } else {  // r = s + (c1 - s)(3x^2 - 4x + 1) + (c2 - s)(3x^2 - 2x)
-      out[f] = (c1[f] - sf[f]) * (q2 - 2.0f * sum + 1.0f) + (c2[f] - sf[f]) * (q2 - sum);
-    }
-  }
-}
-
-template <int METHOD, int NF, class PI, class PO, class SINK>
-HYB_HD void walk_mixed(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
-                       int nf, int kp, uint64_t lcm, int64_t S) {
-  const double thin = kDpthin;
-  const float thinf = (float)kDpthin;
-  auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
-  double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1..l+2) as stored
-  double smin[NF], smax[NF], skp[NF], sfirst[NF], prev[NF], acc[NF];
-  CoefState<float, NF> cst;
-  int xl = 4;
-  auto load = [&](int f, int x) -> double {
-    double v;
-    if (x <= kp) {
-      v = fp[f][(int64_t)(x - 1) * S];
-    } else {
-      v = skp[f];
-    }
-    if (x == kp) skp[f] = v;
-    if (x == 1) {
-      smin[f] = v;
-      smax[f] = v;
-    } else {
-      smin[f] = dmin(smin[f], v);
-      smax[f] = dmax(smax[f], v);
-    }
-    return v;
-  };
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    s0[f] = s1[f] = s2[f] = s3[f] = 0.0;
-    smin[f] = smax[f] = skp[f] = sfirst[f] = prev[f] = acc[f] = 0.0;
-    cst.asc[f] = cst.alc[f] = 0.0f;
-    cst.wc1[f] = cst.wc2[f] = cst.wz1[f] = cst.wz2[f] = cst.zw3l[f] = 0.0f;
-    if (f < nf) {
-      s1[f] = load(f, 1);
-      if (K >= 2) s2[f] = load(f, 2);
-      if (K >= 3) s3[f] = load(f, 3);
-      sfirst[f] = s1[f];
-      if (METHOD == 2) cst.alc[f] = (float)s1[f];
-      if (METHOD == 3) {
-        cst.wc1[f] = (float)s1[f];
-        cst.wc2[f] = (float)s1[f];
-      }
-    }
-  }
-  double pv0 = 0.0, pv1 = pi.first(), pv2 = 0.0, pv3 = 0.0, pv4 = 0.0;
-  if (K + 1 >= 2) pv2 = pi.nextfrom(1, pv1);
-  if (K + 1 >= 3) pv3 = pi.nextfrom(2, pv2);
-  if (K + 1 >= 4) pv4 = pi.nextfrom(3, pv3);
-  uint64_t thick = 0;
-  if (dmax(pv2 - pv1, thin) > thin) thick |= 1ull;
-  if (K >= 2 && dmax(pv3 - pv2, thin) > thin) thick |= 2ull;
-  if (K >= 3 && dmax(pv4 - pv3, thin) > thin) thick |= 4ull;
-  cst.qdpjp_l = 0.0f;
-  cst.wq_l = 0.0f;
-  if (K >= 2) {
-    const float d1 = gmax((float)(pv2 - pv1), thinf), d2 = gmax((float)(pv3 - pv2), thinf);
-    cst.qdpjp_l = 1.0f / (d1 + d2);
-    cst.wq_l = 1.0f / (d1 + d2);
-  }
-  const double zb0 = dmax(po(1), pv1);
-  double zt_c = zb0;  // top of output kc
-  int kc = 1;
-  bool accon = false;  // output kc holds pieces of layers above
-
-  auto emit = [&](const double* v) {
-    bool certain = true;
-#pragma unroll
-    for (int f = 0; f < NF; f++)
-      if (f < nf) {
-        prev[f] = v[f];
-        if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
-      }
-    sink.emit(kc, v, certain, (thick >> (kc - 1)) & 1ull);
-    kc = kc + 1;
-  };
-
-#pragma unroll 1
-  for (int l = 1; l <= K; l++) {
-    const double dprs_m = pv1 - pv0, dprs_0 = pv2 - pv1, dprs_p = pv3 - pv2,
-                 dprs_p2 = pv4 - pv3;
-    const float fm = (float)dprs_m, f0 = (float)dprs_0, fp_ = (float)dprs_p,
-                fp2 = (float)dprs_p2;
-    float c1l[NF], c2l[NF], c3l[NF], sf0[NF], sf1[NF], sf2[NF], sf3[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      sf0[f] = (float)s0[f];
-      sf1[f] = (float)s1[f];
-      sf2[f] = (float)s2[f];
-      sf3[f] = (float)s3[f];
-    }
-    coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lcmb(l), thinf, 1.0e-8f, fm, f0, fp_,
-                                  gmax(fm, thinf), gmax(f0, thinf), gmax(fp_, thinf),
-                                  gmax(fp2, thinf), sf0, sf1, sf2, sf3, cst, c1l, c2l, c3l);
-    const double a = pv1, b = pv2;
-    if (l < K && b >= zx) {  // the walk ends in this layer: complete the range
-      for (int x = xl; x <= K; x++) {
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-          if (f < nf) (void)load(f, x);
-      }
-      xl = K + 1;
-    }
-    const bool active = kc <= K && (accon || zt_c < b || l == K || b >= zx);
-    if (active) {
-      const double dpiv = dmax(b - a, thin);
-      const float rdp = 1.0f / (float)dpiv;
-      const float rw = (b - a > 0.0) ? 1.0f / (float)(b - a) : 0.0f;
-      // pass A: sum of the float deviation integrals of this layer's pieces
-      double esum[NF];
-#pragma unroll
-      for (int f = 0; f < NF; f++) esum[f] = 0.0;
-      PO po0 = po;
-      {
-        double za = accon ? a : zt_c;
-        for (int kq = kc; kq <= K; kq++) {
-          const double zb = dmin(po(kq + 1), zx);
-          const bool ends = (l == K) || (b >= zb);
-          const double ze = ends ? dmax(zb, za) : b;
-          const double w = ze - za;
-          if (w > 0.0) {
-            float dv[NF];
-            avgdev_f<METHOD, NF>(c1l, c2l, c3l, sf1, (float)(za - a) * rdp,
-                                 (float)(ze - a) * rdp, dv);
-#pragma unroll
-            for (int f = 0; f < NF; f++) esum[f] = esum[f] + w * (double)dv[f];
-          }
-          if (!ends) break;
-          za = ze;
-          if (zb >= b && !(l == K || b >= zx)) break;
-        }
-      }
-      po = po0;
-      // pass B: the pieces, corrected so the layer hands out s(l)*dp(l)
-      double za = accon ? a : zt_c;
-      for (; kc <= K;) {
-        const double zt = zt_c;
-        const double zb = dmin(po(kc + 1), zx);
-        const bool ends = (l == K) || (b >= zb);
-        const double ze = ends ? dmax(zb, za) : b;
-        const double w = ze - za;
-        if (w > 0.0) {
-          float dv[NF];
-          avgdev_f<METHOD, NF>(c1l, c2l, c3l, sf1, (float)(za - a) * rdp, (float)(ze - a) * rdp,
-                               dv);
-          const float share = (float)w * rw;
-#pragma unroll
-          for (int f = 0; f < NF; f++)
-            if (f < nf)
-              acc[f] = acc[f] + (s1[f] * w + (w * (double)dv[f] - esum[f] * (double)share));
-        }
-        if (!ends) {
-          accon = true;
-          break;
-        }
-        // output kc is complete (hybgen_*_remap: thin outputs copy the
-        // one above, the first one copies s(1))
-        double v[NF];
-        const bool thn = (zb - zt <= thin) || (zt >= zx);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-          v[f] = 0.0;
-          if (f < nf) v[f] = thn ? ((kc != 1) ? prev[f] : sfirst[f]) : acc[f] / (zb - zt);
-          acc[f] = 0.0;
-        }
-        accon = false;
-        zt_c = zb;
-        emit(v);
-        za = ze;
-        if (zb >= b && !(l == K || b >= zx)) break;
-      }
-    }
-    if (kc > K) break;
-    // slide
-    const double top = pv4;
-    pv0 = pv1;
-    pv1 = pv2;
-    pv2 = pv3;
-    pv3 = top;
-    pv4 = (l + 4 <= K + 1) ? pi.nextfrom(l + 3, top) : 0.0;
-    if (l + 3 <= K && dmax(pv4 - pv3, thin) > thin) thick |= 1ull << (l + 2);
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      s0[f] = s1[f];
-      s1[f] = s2[f];
-      s2[f] = s3[f];
-      s3[f] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
-    }
-    xl++;
-  }
-  sink.fixup(smin, smax);
-}
-
 template <int METHOD, int NF, class PO>
 HYB_HD void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
                         PO& po) {
@@ -1572,7 +1396,7 @@ HYB_HD void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i
     double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
     PiArray pi{pres, S};
     FieldSink<NF> sink{fp, nf, th, P.thbase, S};
-    walk_mixed<METHOD, NF>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
+    walk_core<METHOD, NF, 0, true>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
   }
 }
 
@@ -1661,7 +1485,7 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   if (RECOMP) {
     PoRecompute po{p0, p1, S, depthu};
     UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
-    if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    if (MIXED) walk_core<METHOD, 1, 0, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
 #pragma unroll 2
@@ -1673,7 +1497,7 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   } else {
     PoSum po{dcol};
     UvSink<PoSum> sink{uu, S, ub, pr, &po};
-    if (MIXED) walk_mixed<METHOD, 1>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    if (MIXED) walk_core<METHOD, 1, 0, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
 #pragma unroll 2
     for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);

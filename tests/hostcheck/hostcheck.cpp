@@ -316,7 +316,7 @@ static void column_walk(int kk, const double* si, const double* pi, const double
   PiArray pa{pres, 1};
   LaneRow pr{prsf, 1};
   FieldSink<1> sink{op, 1, nullptr, 0.0, 1};
-  if (MIXED) walk_mixed<M, 1>(kk, pres[kk], pa, pr, sink, fp, 1, kk, lcm, 1);
+  if (MIXED) walk_core<M, 1, 0, true>(kk, pres[kk], pa, pr, sink, fp, 1, kk, lcm, 1);
   else walk_core<M, 1, 0>(kk, pres[kk], pa, pr, sink, fp, 1, kk, lcm, 1);
 }
 
