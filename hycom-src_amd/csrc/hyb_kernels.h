@@ -40,7 +40,7 @@ int uv_method_of(const KParams& P);
 struct StreamTuning {
   int remap_variant = 2;  // 3 fields per walk, no prefetch (fastest measured)
   int uv_prefetch = 2;
-  int uv_recompute = 0;  // 1: u/v new thicknesses formed again from p, no LDS
+  int uv_recompute = 1;  // u/v new thicknesses formed again from p, no LDS (fastest measured)
 };
 // Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
 // plus uvf (2*NIJ int32).

@@ -651,6 +651,36 @@ struct PoSum {
   }
 };
 
+// New interfaces of a p-column read from p (regrid_stream's output) as the
+// walk asks for them, with the next level always loaded ahead.  A walk's
+// requests are k = 1, 2, 2, 3, ... (non-decreasing by steps of one), so
+// the load of level k+1 has a whole layer iteration to arrive; the next
+// walk over the same column starts again at k = 1.
+struct PoAhead {
+  const double* p;  // level 1 of the column
+  int64_t S;
+  int k1;           // kk+1, the deepest level
+  int k = 1;        // level of cur
+  double cur = 0.0, nxt = 0.0;
+  HYB_HD PoAhead(const double* p_, int64_t S_, int kk) : p(p_), S(S_), k1(kk + 1) {
+    cur = p[0];
+    nxt = p[S];
+  }
+  HYB_HD double operator()(int kq) {
+    if (kq < k) {
+      k = kq;
+      cur = p[(int64_t)(k - 1) * S];
+      nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
+    }
+    while (k < kq) {
+      cur = nxt;
+      k++;
+      nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
+    }
+    return cur;
+  }
+};
+
 // The same prsf(k) without the LDS column: the new thicknesses are formed
 // again from p at the two neighbouring p-points as the walk asks for them
 // (the operations of the first pass of uv_stream, so the same bits).

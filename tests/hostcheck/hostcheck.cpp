@@ -162,7 +162,7 @@ extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, i
 #include "hyb_stream.h"
 
 static int64_t g_unsafe_cols = 0;
-static int g_uv_recompute = 0;  // 1: u/v walks without the LDS column
+static int g_uv_recompute = 0;  // 1: remap and u/v walks without the LDS column
 static int64_t g_uv_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
@@ -203,6 +203,16 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
           case 1: remap_mixed<1, 3>(P, V, W, i, j, n, po); break;
           case 2: remap_mixed<2, 3>(P, V, W, i, j, n, po); break;
           default: remap_mixed<3, 3>(P, V, W, i, j, n, po); break;
+        }
+        continue;
+      }
+      if (g_uv_recompute) {  // the variants that read the new interfaces from p
+        PoAhead pa(V.p + c0, V.NIJ, kk);
+        switch (P.hybmap) {
+          case 0: remap_stream<0, 2, 0>(P, V, W, i, j, n, pa); break;
+          case 1: remap_stream<1, 2, 0>(P, V, W, i, j, n, pa); break;
+          case 2: remap_stream<2, 3, 0>(P, V, W, i, j, n, pa); break;
+          default: remap_stream<3, 2, 2>(P, V, W, i, j, n, pa); break;
         }
         continue;
       }
@@ -284,7 +294,7 @@ extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F,
   return 0;
 }
 
-// the streaming family with the LDS-free u/v variant
+// the streaming family with the LDS-free remap and u/v variants
 extern "C" int hostcheck_hybgen_stream_rc(const hyb_params* P, const hyb_fields* F, int m,
                                           int n, int raflag) {
   g_uv_recompute = 1;

@@ -156,9 +156,12 @@ def test_streaming_any_kdm(gpu, kdm, hybmap, ntracr, extreme):
 @pytest.mark.parametrize("hybmap", [1, 2, 3])
 def test_mixed_precision_on_gpu(gpu, hybmap):
     """HYB_PREC_MIXED on the GPU: interfaces bitwise the FP64 oracle's, every
-    remapped field within 1e-5 of its scale, and within 1e-6 of the host
-    build of the same mixed code (the device uses the hardware reciprocals,
-    the host IEEE quotients)."""
+    remapped field within 1e-5 of its scale, and within 1e-5 of the host
+    build of the same mixed code.  The device uses the hardware reciprocals
+    and the host IEEE quotients; a limiter comparison that lands on the
+    other side of its threshold moves a value by more than the rounding
+    alone, so the two builds are held to the stated tolerance, not to
+    the ulp."""
     import hostcheck_lib as hc
     P, w = _case("mixed", idm=90, jdm=57, kdm=41, hybmap=hybmap, ntracr=3, trcflg=[2, 0, 0])
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=9, extreme=1))
@@ -180,13 +183,13 @@ def test_mixed_precision_on_gpu(gpu, hybmap):
     for f in ("temp", "saln", "th3d", "tracer", "u", "v"):
         x = host.a[f]
         err = np.max(np.abs(x - got.a[f])) / np.max(np.abs(x))
-        assert err <= 1e-6, f"{f}: GPU mixed vs host build of the same code {err:.2e}"
+        assert err <= 1e-5, f"{f}: GPU mixed vs host build of the same code {err:.2e}"
 
 
 @pytest.mark.parametrize("kdm,hybmap,extreme", [(41, 2, 0), (22, 1, 1), (64, 0, 1), (17, 3, 1)])
-def test_streaming_uv_without_lds(gpu, kdm, hybmap, extreme):
-    """The u/v walk that forms its new thicknesses again from p instead of
-    keeping them in LDS (tuning knob 2) gives the oracle's bits."""
+def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
+    """The u/v walk that keeps its new thicknesses in LDS (tuning knob 2 =
+    0; the default forms them again from p) gives the oracle's bits."""
     dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
     P, w = _case("uv-recompute", idm=90, jdm=57, kdm=kdm, hybmap=hybmap, nsigma=min(8, kdm),
                  hybiso=3.0 if hybmap == 1 else 0.01, **dx)
@@ -195,11 +198,33 @@ def test_streaming_uv_without_lds(gpu, kdm, hybmap, extreme):
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm + 5, extreme=extreme))
     want = _oracle_out(d0)
     dd = abi.DeviceDomain(P).upload(d0)
-    abi.check(lib.hycom_hybgen_tune(2, 1))
+    abi.check(lib.hycom_hybgen_tune(2, 0))
     try:
         dd.hybgen(m=1, n=2)
         got = dd.download()
     finally:
-        abi.check(lib.hycom_hybgen_tune(2, 0))
+        abi.check(lib.hycom_hybgen_tune(2, 1))
         abi.check(lib.hycom_hybgen_set_kernel_path(0))
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
+
+
+@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 2])
+def test_streaming_remap_variants(gpu, variant):
+    """Every p-column remap variant (fields per walk, prefetch, new
+    interfaces from LDS or read ahead from p, occupancy target) gives the
+    oracle's bits, fallback columns included."""
+    P, w = _case("remap-variant", idm=90, jdm=57, kdm=41, hybmap=2, ntracr=5,
+                 trcflg=[2, 0, 0, 0, 0], dx00=30.0, dx00f=1.1, dx00x=300.0)
+    _select(3, P)
+    lib = abi.load()
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=61, extreme=1))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    abi.check(lib.hycom_hybgen_tune(0, variant))
+    try:
+        dd.hybgen(m=1, n=2)
+        got = dd.download()
+    finally:
+        abi.check(lib.hycom_hybgen_tune(0, 2))
+        abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    _assert_same(want, got, f"remap variant {variant}")
