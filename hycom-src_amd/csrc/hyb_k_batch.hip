@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   const double* pg = V.p + c0;
   for (int k = 1; k <= P.kk + 1; k++) row[(k - 1) * CW] = pg[(int64_t)(k - 1) * V.NIJ];
   LaneRow po{row, CW};
-  if (MIXED) remap_mixed<METHOD, 3>(P, V, W, i, r.j, n, po);
+  if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
   else remap_stream<METHOD, 3, 0>(P, V, W, i, r.j, n, po);
 }
 
@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int 
   const int64_t c0 = ij_of(V, i, j);
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
-  LaneRow dcol{lds + threadIdx.x, kWave};
-  const bool ok = uv_stream<METHOD, MIXED ? 0 : 2, MIXED>(P, V, i, j, nl, isv, dcol);
+  LaneRow dcol{lds + threadIdx.x, kWave};  // unused: new thicknesses formed again from p
+  const bool ok = uv_stream<METHOD, 2, MIXED, true>(P, V, i, j, nl, isv, dcol);
   B.W[d].uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
 }
 
@@ -199,8 +199,7 @@ hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipSt
     hipLaunchKernelGGL(kb_halo_j, dim3(L.halo_j.nblocks), dim3(kBlock), 0, s, B, L.halo_j);
   if (L.halo_i.nblocks > 0)
     hipLaunchKernelGGL(kb_halo_i, dim3(L.halo_i.nblocks), dim3(kBlock), 0, s, B, L.halo_i);
-  const size_t lds_u = sizeof(double) * L.kmax * kWave;
-  for (int q = 0; q < L.nuv; q++) uv_group(B, L.uv[q], n, lds_u, s);
+  for (int q = 0; q < L.nuv; q++) uv_group(B, L.uv[q], n, 0, s);
   if (L.uvall.nblocks > 0)
     hipLaunchKernelGGL(kb_uv_fb, dim3(L.uvall.nblocks), dim3(kWave), 0, s, B, L.uvall, n);
   return hipGetLastError();

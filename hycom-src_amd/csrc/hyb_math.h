@@ -13,6 +13,10 @@
 #else
 #define HYB_HD inline
 #endif
+// the per-column bodies: inlined into every kernel that runs them, so a
+// kernel never pays a device-function call (stack, argument registers,
+// generic loads of the parameter block) for the hot path
+#define HYB_HDI HYB_HD __attribute__((always_inline))
 
 namespace hyb {
 

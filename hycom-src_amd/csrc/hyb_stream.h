@@ -76,7 +76,7 @@ HYB_HD void shl(double (&a)[W], double v) {
 // ---------------------------------------------------------------------
 // regrid of one p-column (hybgenaj up to the remap)
 // ---------------------------------------------------------------------
-HYB_HD void regrid_stream(const KParams& P, const View& V, const Scratch& W, int i, int j,
+HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, int i, int j,
                           int n) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
@@ -689,23 +689,32 @@ struct PoRecompute {
   const double* p1;
   int64_t S;
   double depthu;
+  int k1;                                // kk+1, the deepest level
   double prv = 0.0, cur = 0.0, b = 0.0;  // prsf(k-1), prsf(k), min(depthu, p-mean at k)
-  int k = 1;
+  double q0 = 0.0, q1 = 0.0;             // p at level k+1 of both points, loaded ahead
+  int k = 0;                             // 0 until the first request
+  HYB_HD PoRecompute(const double* a0, const double* a1, int64_t s, double du, int kk)
+      : p0(a0), p1(a1), S(s), depthu(du), k1(kk + 1) {}
   HYB_HD double operator()(int kq) {
-    if (kq == 1) {
+    if (kq == 1 || k == 0) {
       prv = 0.0;
       cur = 0.0;
       b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
       k = 1;
-      return cur;
+      q0 = p0[S];
+      q1 = p1[S];
     }
     while (k < kq) {
-      const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
+      const double a = dmin(depthu, 0.5 * (q0 + q1));
       const double d = dmax(0.0, a - b);
       b = a;
       prv = cur;
       cur = cur + d;
       k++;
+      if (k < k1) {
+        q0 = p0[(int64_t)k * S];
+        q1 = p1[(int64_t)k * S];
+      }
     }
     return cur;
   }
@@ -1003,7 +1012,7 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
 // interfaces) are loaded ahead of the window, to cover memory latency.
 // ---------------------------------------------------------------------
 template <int METHOD, int NF, int PD, bool FAST = false, class PI, class PO, class SINK>
-HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
+HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
                       int nf, int kp, uint64_t lcm, int64_t S) {
   constexpr int PQ = PD > 0 ? PD : 1;
   using CR = typename std::conditional<FAST, float, double>::type;
@@ -1021,13 +1030,10 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
   int xl = 4;                                  // next level to load
   // s(x) with the massless fill (972-996) and the running min/max
   // (e.g. 2727-2731: si_min/si_max over k = 1..kk in order)
-  auto load = [&](int f, int x) -> double {
-    double v;
-    if (x <= kp) {
-      v = fp[f][(int64_t)(x - 1) * S];
-    } else {
-      v = skp[f];
-    }
+  // fold(f, x, r): level x enters the column in order, r being its stored
+  // value (ignored below kp); idempotent, so a level may be folded again
+  auto fold = [&](int f, int x, double r) -> double {
+    const double v = (x <= kp) ? r : skp[f];
     if (x == kp) skp[f] = v;
     if (x == 1) {
       smin[f] = v;
@@ -1037,6 +1043,16 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
       smax[f] = dmax(smax[f], v);
     }
     return v;
+  };
+  auto load = [&](int f, int x) -> double {
+    return fold(f, x, (x <= kp) ? fp[f][(int64_t)(x - 1) * S] : 0.0);
+  };
+  // the stored value of level min(x, kp), loaded without a use: with
+  // PD > 0 it is folded PD layer iterations later, so the load has that
+  // long to arrive (its value is discarded below kp)
+  auto raw = [&](int f, int x) -> double {
+    const int xc = x < kp ? x : kp;
+    return fp[f][(int64_t)(xc - 1) * S];
   };
 #pragma unroll
   for (int f = 0; f < NF; f++) {
@@ -1051,7 +1067,7 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
       if (K >= 3) s3[f] = load(f, 3);
       sfirst[f] = s1[f];
 #pragma unroll
-      for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? load(f, 4 + q) : 0.0;
+      for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? raw(f, 4 + q) : 0.0;
       if (METHOD == 2) cst.alc[f] = (CR)s1[f];  // al(1) = s(1)
       if (METHOD == 3) {                        // first sweep of layer 1
         cst.wc1[f] = (CR)s1[f];
@@ -1169,6 +1185,15 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
     // bottom, and the walk stops here: fold the levels not read yet into
     // the column range now, before any of them is overwritten.
     if (l < K && pil1 >= zx) {
+#pragma unroll
+      for (int q = 0; q < PD; q++) {  // levels loaded ahead, not folded yet
+        const int x = l + 3 + q;
+        if (x <= K && x < xl) {
+#pragma unroll
+          for (int f = 0; f < NF; f++)
+            if (f < nf) (void)fold(f, x, sq[f][q]);
+        }
+      }
       for (int x = xl; x <= K; x++) {
 #pragma unroll
         for (int f = 0; f < NF; f++)
@@ -1376,10 +1401,10 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
       if (PD == 0) {
         s3[f] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
       } else {
-        s3[f] = sq[f][0];
+        s3[f] = (f < nf && l + 3 <= K) ? fold(f, l + 3, sq[f][0]) : 0.0;
 #pragma unroll
         for (int q = 0; q < PD - 1; q++) sq[f][q] = sq[f][q + 1];
-        sq[f][PQ - 1] = (f < nf && xl <= K) ? load(f, xl) : 0.0;
+        sq[f][PQ - 1] = (f < nf && xl <= K) ? raw(f, xl) : 0.0;
       }
     }
     xl++;
@@ -1399,8 +1424,8 @@ HYB_HD void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double
 // The overturn/unmix, the interfaces and every decision of the regrid stay
 // FP64 and bitwise the reference's.
 // ---------------------------------------------------------------------
-template <int METHOD, int NF, class PO>
-HYB_HD void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
+template <int METHOD, int NF, int PD, class PO>
+HYB_HDI void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
                         PO& po) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
@@ -1426,12 +1451,12 @@ HYB_HD void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i
     double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
     PiArray pi{pres, S};
     FieldSink<NF> sink{fp, nf, th, P.thbase, S};
-    walk_core<METHOD, NF, 0, true>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
+    walk_core<METHOD, NF, PD, true>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
   }
 }
 
 template <int METHOD, int NF, int PD, class PO>
-HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
+HYB_HDI void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
                          PO& po) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
@@ -1471,7 +1496,7 @@ HYB_HD void remap_stream(const KParams& P, const View& V, const Scratch& W, int 
 // writes pu only and returns false; point_c then redoes the point.
 // ---------------------------------------------------------------------
 template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>
-HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bool isv,
+HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bool isv,
                       const LaneRow& dcol) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
@@ -1513,10 +1538,9 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   PiSum pi{dpu, S};
   double* fp[1] = {uu};
   if (RECOMP) {
-    PoRecompute po{p0, p1, S, depthu};
+    PoRecompute po(p0, p1, S, depthu, K);
     UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
-    if (MIXED) walk_core<METHOD, 1, 0, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
-    else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
 #pragma unroll 2
     for (int k = 1; k <= K; k++) {
@@ -1527,8 +1551,7 @@ HYB_HD bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, boo
   } else {
     PoSum po{dcol};
     UvSink<PoSum> sink{uu, S, ub, pr, &po};
-    if (MIXED) walk_core<METHOD, 1, 0, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
-    else walk_core<METHOD, 1, PD>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
 #pragma unroll 2
     for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
   }

@@ -199,10 +199,10 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
       for (int k = 1; k <= kk + 1; k++) row[k - 1] = V.p[c0 + (int64_t)(k - 1) * V.NIJ];
       if (P.precision == 1) {
         switch (P.hybmap) {
-          case 0: remap_mixed<0, 3>(P, V, W, i, j, n, po); break;
-          case 1: remap_mixed<1, 3>(P, V, W, i, j, n, po); break;
-          case 2: remap_mixed<2, 3>(P, V, W, i, j, n, po); break;
-          default: remap_mixed<3, 3>(P, V, W, i, j, n, po); break;
+          case 0: remap_mixed<0, 3, 2>(P, V, W, i, j, n, po); break;
+          case 1: remap_mixed<1, 3, 2>(P, V, W, i, j, n, po); break;
+          case 2: remap_mixed<2, 3, 2>(P, V, W, i, j, n, po); break;
+          default: remap_mixed<3, 3, 2>(P, V, W, i, j, n, po); break;
         }
         continue;
       }
@@ -259,9 +259,9 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
           bool ok;
           if (P.precision == 1) {
             switch (uv_method(P)) {
-              case 0: ok = uv_stream<0, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
-              case 1: ok = uv_stream<1, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
-              default: ok = uv_stream<3, 0, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 0: ok = uv_stream<0, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 1: ok = uv_stream<1, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              default: ok = uv_stream<3, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
             }
           } else if (g_uv_recompute) {
             switch (uv_method(P)) {
