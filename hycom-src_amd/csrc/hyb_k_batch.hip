@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   for (int k = 1; k <= P.kk + 1; k++) row[(k - 1) * CW] = pg[(int64_t)(k - 1) * V.NIJ];
   LaneRow po{row, CW};
   if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
-  else remap_stream<METHOD, 3, 0>(P, V, W, i, r.j, n, po);
+  else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
 }
 
 __global__ void __launch_bounds__(kWave) kb_remap_fb(BatchDevice B, BatchGroup G, int n) {

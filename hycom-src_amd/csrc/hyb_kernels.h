@@ -38,7 +38,7 @@ int uv_method_of(const KParams& P);
 // remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
 // uv_prefetch the prefetch depth of the u/v walk (0 or 2).
 struct StreamTuning {
-  int remap_variant = 2;  // 3 fields per walk, no prefetch (fastest measured)
+  int remap_variant = 3;  // 3 fields per walk, 2 levels loaded ahead (fastest measured)
   int uv_prefetch = 2;
   int uv_recompute = 1;  // u/v new thicknesses formed again from p, no LDS (fastest measured)
 };

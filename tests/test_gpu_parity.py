@@ -208,7 +208,7 @@ def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
 
 
-@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 2])
+@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 0, 1, 2, 4, 5, 3])
 def test_streaming_remap_variants(gpu, variant):
     """Every p-column remap variant (fields per walk, prefetch, new
     interfaces from LDS or read ahead from p, occupancy target) gives the
@@ -225,6 +225,6 @@ def test_streaming_remap_variants(gpu, variant):
         dd.hybgen(m=1, n=2)
         got = dd.download()
     finally:
-        abi.check(lib.hycom_hybgen_tune(0, 2))
+        abi.check(lib.hycom_hybgen_tune(0, 3))
         abi.check(lib.hycom_hybgen_set_kernel_path(0))
     _assert_same(want, got, f"remap variant {variant}")
