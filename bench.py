@@ -26,6 +26,12 @@ import hybgen_abi as abi  # noqa: E402
 METRIC = "column-layers remapped/s, FP64 vs mixed, GLBb tile; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
 PHASES = ["column", "halo", "uv", "ra", "halo2", "uv2", "regrid", "remap"]
+# kernel of each p-column remap variant (hycom_hybgen_tune knob 0)
+REMAP_KERNELS = {0: "k_remap_s<{m},6,0>", 1: "k_remap_s<{m},6,2>", 2: "k_remap_s<{m},3,0>",
+                 3: "k_remap_s<{m},3,2>", 4: "k_remap_s<{m},2,2>", 5: "k_remap_s<{m},2,4>",
+                 6: "k_remap_g<{m},3,0,1>", 7: "k_remap_g<{m},3,0,3>", 8: "k_remap_g<{m},2,0,3>",
+                 9: "k_remap_g<{m},2,2,3>", 10: "k_remap_g<{m},2,0,4>", 11: "k_remap_g<{m},3,2,1>",
+                 12: "k_remap_g<{m},3,2,3>", 13: "k_remap_g<{m},2,2,4>"}
 
 
 def bytes_per_column_phase_a(K, T):
@@ -155,16 +161,19 @@ def batch_leg(args):
     print(f"[bench] batch of {len(doms)} domains, {layers} sea column-layers",
           file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream().cuda_stream
-    # one call per domain
-    for dd in doms:
-        dd.hybgen(m=1, n=2, stream=stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    t_single = None
+    if args.batch_baseline:
+        # one hycom_hybgen_device call per domain (off by default: these
+        # calls would mix other shapes into the full-domain kernels' traces)
         for dd in doms:
             dd.hybgen(m=1, n=2, stream=stream)
-    torch.cuda.synchronize()
-    t_single = (time.perf_counter() - t0) / args.steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for dd in doms:
+                dd.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        t_single = (time.perf_counter() - t0) / args.steps
     # the batch: autotune, then time
     abi.check(lib.hycom_hybgen_batch_autotune(1))
     batch = abi.Batch(doms)
@@ -184,16 +193,17 @@ def batch_leg(args):
         lat.append(time.perf_counter() - ts)
     t_batch = (time.perf_counter() - t0) / args.steps
     rc, gc, tuned = batch.config()
-    print(f"[bench] batch {1e3 * t_batch:.2f} ms/call, one call per domain "
-          f"{1e3 * t_single:.2f} ms; widths remap {rc} regrid {gc}", file=sys.stderr, flush=True)
+    print(f"[bench] batch {1e3 * t_batch:.2f} ms/call; widths remap {rc} regrid {gc}",
+          file=sys.stderr, flush=True)
     lat.sort()
     shapes = [f"{dd.P.idm}x{dd.P.jdm}x{dd.P.kdm}" for dd in doms]
     del batch, doms
     torch.cuda.empty_cache()
     return {"value": layers / t_batch, "unit": "column-layers/s", "ms_per_call": 1e3 * t_batch,
             "domains": len(shapes), "sea_column_layers": layers,
-            "one_call_per_domain": {"value": layers / t_single, "ms_per_step": 1e3 * t_single},
-            "speedup_vs_one_call_per_domain": t_single / t_batch,
+            "one_call_per_domain": ({"value": layers / t_single, "ms_per_step": 1e3 * t_single,
+                                     "speedup_of_batch": t_single / t_batch}
+                                    if t_single else None),
             "autotuned": {"remap_cols": rc, "regrid_cols": gc, "tuned": tuned,
                           "tuning_calls": calls},
             "latency_ms": {"p50": 1e3 * lat[len(lat) // 2], "p99": 1e3 * lat[-1]},
@@ -292,6 +302,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched ragged domains leg")
     ap.add_argument("--batch-domains", type=int, default=64)
+    ap.add_argument("--batch-baseline", action="store_true",
+                    help="also time the batch's domains one hycom_hybgen_device call each")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop leg")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
@@ -299,8 +311,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--kernel-path", type=int, default=0,
                     help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
-    ap.add_argument("--remap-variant", type=int, default=None,
-                    help="streaming remap variant 0..5 (hycom_hybgen_tune knob 0)")
+    ap.add_argument("--remap-variant", type=int, default=3,
+                    help="streaming remap variant 0..13 (hycom_hybgen_tune knob 0; 3 is "
+                         "the library default)")
     ap.add_argument("--uv-prefetch", type=int, default=None,
                     help="streaming u/v prefetch depth 0 or 2 (knob 1)")
     ap.add_argument("--uv-recompute", type=int, default=None,
@@ -316,8 +329,7 @@ def main():
     import torch
     lib = abi.load()
     abi.check(lib.hycom_hybgen_set_kernel_path(args.kernel_path))
-    if args.remap_variant is not None:
-        abi.check(lib.hycom_hybgen_tune(0, args.remap_variant))
+    abi.check(lib.hycom_hybgen_tune(0, args.remap_variant))
     if args.uv_prefetch is not None:
         abi.check(lib.hycom_hybgen_tune(1, args.uv_prefetch))
     if args.uv_recompute is not None:
@@ -362,7 +374,7 @@ def main():
     value = sea * P.kdm * args.steps / total
     if cnt[7]:
         # the streaming kernels: the p-column remap kernel on its own
-        kname = f"k_remap_s<{args.hybmap},3,0> (p-column remap, hybgenaj)"
+        kname = f"{REMAP_KERNELS[args.remap_variant].format(m=args.hybmap)} (p-column remap, hybgenaj)"
         ms_a = ms[7] / cnt[7]
         bpc_a = bytes_per_column_remap(P.kdm, P.ntracr)
     else:

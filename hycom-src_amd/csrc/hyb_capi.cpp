@@ -114,10 +114,15 @@ void batch_layout(const KParams* hP, int nd, BatchLaunch& L, std::vector<int32_t
     G->dom = (const int32_t*)(uintptr_t)od;
     G->prefix = (const int32_t*)(uintptr_t)op;
   };
+  // row kernels map rows as i_aligned0 does: kAlignPad more columns a row
   auto rows = [](int cols) {
-    return [cols](const KParams& P) { return (int64_t)P.jj * ((P.ii + cols - 1) / cols); };
+    return [cols](const KParams& P) {
+      return (int64_t)P.jj * ((P.ii + kAlignPad + cols - 1) / cols);
+    };
   };
-  auto uvblocks = [](const KParams& P) { return 2 * (int64_t)P.jj * ((P.ii + 63) / 64); };
+  auto uvblocks = [](const KParams& P) {
+    return 2 * (int64_t)P.jj * ((P.ii + kAlignPad + 63) / 64);
+  };
   std::vector<int> all(nd);
   for (int d = 0; d < nd; d++) all[d] = d;
   L.kmax = 0;

@@ -41,6 +41,17 @@ struct View {
 HYB_HD int64_t ij_of(const View& V, int i, int j) {
   return (int64_t)(i - 1 + V.nb) + V.NI * (int64_t)(j - 1 + V.nb);
 }
+// First i of a row's lane mapping with 128-byte-aligned wave rows: lane 0
+// of the row's first block sits on a 128-byte boundary of the row (of
+// every level when NI*NJ is a multiple of 16 and the array is 128-byte
+// aligned), so the 64 columns of a wave are four whole cache lines instead
+// of five partial ones.  Kernels mapped this way launch
+// ceil((ii + kAlignPad) / width) blocks per row and skip i < 1 and i > ii.
+constexpr int kAlignPad = 15;
+HYB_HD int i_aligned0(const View& V, int j) {
+  return 1 - (int)(((int64_t)V.nb + V.NI * (int64_t)(j - 1 + V.nb)) & 15);
+}
+
 // offsets of level k (1-based), time level l, tracer t
 HYB_HD int64_t lev(const View& V, int k) { return V.NIJ * (int64_t)(k - 1); }
 HYB_HD int64_t lev4(const View& V, int k, int l) {

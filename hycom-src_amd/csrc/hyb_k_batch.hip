@@ -21,25 +21,28 @@ __device__ __forceinline__ int group_of(const int32_t* prefix, int ng, int b) {
   return lo;
 }
 
-// block -> (domain, j, first i) for a row kernel with `bw` threads per block
+// block -> (domain, j, first i) for a row kernel with `bw` threads per
+// block, rows mapped as i_aligned0 maps them
 struct RowPos {
   int d, j, i0;
 };
-__device__ __forceinline__ RowPos row_pos(const BatchGroup& G, const KParams* Ps, int bw, int b) {
+__device__ __forceinline__ RowPos row_pos(const BatchDevice& B, const BatchGroup& G, int bw,
+                                          int b) {
   const int g = group_of(G.prefix, G.ndom, b);
   const int d = G.dom[g];
   const int local = b - G.prefix[g];
-  const int nbi = (Ps[d].ii + bw - 1) / bw;
-  return RowPos{d, local / nbi + 1, (local % nbi) * bw};
+  const int nbi = (B.P[d].ii + kAlignPad + bw - 1) / bw;
+  const int j = local / nbi + 1;
+  return RowPos{d, j, i_aligned0(B.V[d], j) + (local % nbi) * bw};
 }
 
 template <int BW>
 __global__ void __launch_bounds__(BW) kb_regrid(BatchDevice B, BatchGroup G, int n) {
-  const RowPos r = row_pos(G, B.P, BW, blockIdx.x);
+  const RowPos r = row_pos(B, G, BW, blockIdx.x);
   const KParams& P = B.P[r.d];
   const View& V = B.V[r.d];
-  const int i = r.i0 + threadIdx.x + 1;
-  if (i > P.ii) return;
+  const int i = r.i0 + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, r.j)] == 0) return;
   regrid_stream(P, V, B.W[r.d], i, r.j, n);
 }
@@ -48,12 +51,12 @@ __global__ void __launch_bounds__(BW) kb_regrid(BatchDevice B, BatchGroup G, int
 template <int METHOD, bool MIXED, int CW>
 __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int n) {
   extern __shared__ double lds[];
-  const RowPos r = row_pos(G, B.P, CW, blockIdx.x);
+  const RowPos r = row_pos(B, G, CW, blockIdx.x);
   const KParams& P = B.P[r.d];
   const View& V = B.V[r.d];
   const Scratch& W = B.W[r.d];
-  const int i = r.i0 + threadIdx.x + 1;
-  if (i > P.ii) return;
+  const int i = r.i0 + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, r.j);
   if (V.ip[c0] == 0) return;
   if (W.kp[c0] & kUnsafe) return;
@@ -66,12 +69,12 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
 }
 
 __global__ void __launch_bounds__(kWave) kb_remap_fb(BatchDevice B, BatchGroup G, int n) {
-  const RowPos r = row_pos(G, B.P, kWave, blockIdx.x);
+  const RowPos r = row_pos(B, G, kWave, blockIdx.x);
   const KParams& P = B.P[r.d];
   const View& V = B.V[r.d];
   const Scratch& W = B.W[r.d];
-  const int i = r.i0 + threadIdx.x + 1;
-  if (i > P.ii) return;
+  const int i = r.i0 + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, r.j);
   if (V.ip[c0] == 0) return;
   if (!(W.kp[c0] & kUnsafe)) return;
@@ -111,13 +114,13 @@ __global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int 
   const int d = G.dom[g];
   const KParams& P = B.P[d];
   const View& V = B.V[d];
-  const int nbi = (P.ii + kWave - 1) / kWave;
+  const int nbi = (P.ii + kAlignPad + kWave - 1) / kWave;
   int local = blockIdx.x - G.prefix[g];
   const bool isv = local >= P.jj * nbi;
   if (isv) local -= P.jj * nbi;
   const int j = local / nbi + 1;
-  const int i = (local % nbi) * kWave + threadIdx.x + 1;
-  if (i > P.ii) return;
+  const int i = i_aligned0(V, j) + (local % nbi) * kWave + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, j);
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
@@ -131,13 +134,13 @@ __global__ void __launch_bounds__(kWave) kb_uv_fb(BatchDevice B, BatchGroup G, i
   const int d = G.dom[g];
   const KParams& P = B.P[d];
   const View& V = B.V[d];
-  const int nbi = (P.ii + kWave - 1) / kWave;
+  const int nbi = (P.ii + kAlignPad + kWave - 1) / kWave;
   int local = blockIdx.x - G.prefix[g];
   const bool isv = local >= P.jj * nbi;
   if (isv) local -= P.jj * nbi;
   const int j = local / nbi + 1;
-  const int i = (local % nbi) * kWave + threadIdx.x + 1;
-  if (i > P.ii) return;
+  const int i = i_aligned0(V, j) + (local % nbi) * kWave + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, j);
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;

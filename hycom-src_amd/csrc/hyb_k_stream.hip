@@ -6,9 +6,9 @@
 namespace hyb {
 
 __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch W, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int j = blockIdx.y + 1;
-  if (i > P.ii) return;
+  int i = i_aligned0(V, j) + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, j)] == 0) return;
   regrid_stream(P, V, W, i, j, n);
 }
@@ -16,10 +16,10 @@ __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch 
 template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>
 __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, int nl) {
   extern __shared__ double lds[];
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int j = blockIdx.y + 1;
+  int i = i_aligned0(V, j) + blockIdx.x * blockDim.x + threadIdx.x;
   bool isv = blockIdx.z != 0;
-  if (i > P.ii) return;
+  if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, j);
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
@@ -32,7 +32,7 @@ template <int M, bool RECOMP>
 static void uv_stream_m(const KParams& P, const View& V, const Scratch& W, int nl, int pd,
                         hipStream_t s) {
   dim3 w1(kWave);
-  dim3 gw3((P.ii + kWave - 1) / kWave, P.jj, 2);
+  dim3 gw3((P.ii + kAlignPad + kWave - 1) / kWave, P.jj, 2);
   const size_t lds = RECOMP ? 0 : sizeof(double) * P.kk * kWave;
   if (P.precision == 1)
     hipLaunchKernelGGL((k_uv_s<M, 2, true, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
@@ -68,7 +68,7 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   if (T)
     for (int q = 0; q < kPhases; q++) T->used[q] = false;
   dim3 blk(kBlock);
-  dim3 g2((P.ii + kBlock - 1) / kBlock, P.jj);
+  dim3 g2((P.ii + kAlignPad + kBlock - 1) / kBlock, P.jj);
   launch_diag_pre(P, V, n, s);
   {
     PhaseMark pm(T, s, 0);

@@ -118,10 +118,16 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   pres[S] = pk1;
   int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
   int kp = 2;
+  // dp(k..k+3) loaded ahead: each level has three iterations to arrive
+  double dq[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) dq[q] = FK(dpc, (2 + q <= K) ? 2 + q : K);
 #pragma unroll 1
   for (int k = 2; k <= K; k++) {
+    const double dk = dq[0];
+    shl(dq, FK(dpc, (k + 4 <= K) ? k + 4 : K));
     pk = pk1;
-    pk1 = pk + FK(dpc, k);
+    pk1 = pk + dk;
     pres[(int64_t)k * S] = pk1;
     if (k >= 3 && pk1 - pk >= dpthin) kp = k;  // 590-598: the deepest such k
     double dtf = 0.0, cumk1;
@@ -362,7 +368,8 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   double pf = 0.0, prs = 0.0;
   double* pg = V.p + c0;
   bool unsafe = false;
-  auto consume = [&](int x, double v) {
+  // pchk: pres(x-3) as left by the overturn, for the in-place check
+  auto consume = [&](int x, double v, double pchk) {
     if (x == 1) {
       pf = v;
       prs = v;
@@ -383,13 +390,15 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     // that reaches the bottom, only if pres(jx-2) >= zb = min(prsf(jx+1),zx)
     if (jx >= 4) {
       const double zb = dmin(prs, zx);
-      if (zb < zx && pres[(int64_t)(jx - 3) * S] >= zb) unsafe = true;
+      if (zb < zx && pchk >= zb) unsafe = true;
     }
   };
 
   double q, p_hat, p_hat0, p_hat2, p_hat3;
 #pragma unroll 1
   for (int k = 2; k <= K; k++) {
+    // pres(k-5) for the in-place check of consume(k-2), issued early
+    const double pchk = (k >= 7) ? pres[(int64_t)(k - 6) * S] : 0.0;
 #define PW(x) pw[(x) - (k - 3)]
     const double dm1 = dw[2], dm2 = dw[1], dm3 = dw[0], d0 = dw[3], dp1 = dw[4];
     const double qm1 = qw[1], qm2 = qw[0], q0 = qw[2], qp1 = qw[3];
@@ -566,7 +575,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     }
 #undef PW
     // p(k-2) is final now
-    if (k >= 3) consume(k - 2, pw[1]);
+    if (k >= 3) consume(k - 2, pw[1], pchk);
     // slide: level k+4 enters pw, k+3 enters tw/thw/lcm, k+2 enters dw/qw
     const double r4 = raw_at(k + 4);
     shl(pw, (k + 4 <= K + 1) ? with_floor(k + 4, r4) : 0.0);
@@ -592,9 +601,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     shl(rw, r4);
   }
   // levels kk-1, kk, kk+1 (pw holds p(kk-2..kk+4) now)
-  consume(K - 1, pw[1]);
-  consume(K, pw[2]);
-  consume(K + 1, pw[3]);
+  consume(K - 1, pw[1], K - 1 >= 5 ? pres[(int64_t)(K - 5) * S] : 0.0);
+  consume(K, pw[2], K >= 5 ? pres[(int64_t)(K - 4) * S] : 0.0);
+  consume(K + 1, pw[3], K + 1 >= 5 ? pres[(int64_t)(K - 3) * S] : 0.0);
   W.kp[c0] = kp | (unsafe ? kUnsafe : 0);
   W.lcm[c0] = lcm;
   if (V.rec_fixlay) {
