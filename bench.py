@@ -31,7 +31,8 @@ REMAP_KERNELS = {0: "k_remap_s<{m},6,0>", 1: "k_remap_s<{m},6,2>", 2: "k_remap_s
                  3: "k_remap_s<{m},3,2>", 4: "k_remap_s<{m},2,2>", 5: "k_remap_s<{m},2,4>",
                  6: "k_remap_g<{m},3,0,1>", 7: "k_remap_g<{m},3,0,3>", 8: "k_remap_g<{m},2,0,3>",
                  9: "k_remap_g<{m},2,2,3>", 10: "k_remap_g<{m},2,0,4>", 11: "k_remap_g<{m},3,2,1>",
-                 12: "k_remap_g<{m},3,2,3>", 13: "k_remap_g<{m},2,2,4>"}
+                 12: "k_remap_g<{m},3,2,3>", 13: "k_remap_g<{m},2,2,4>",
+                 14: "k_remap_s<{m},3,4>", 15: "k_remap_n<{m},3,2>"}
 
 
 def bytes_per_column_phase_a(K, T):

@@ -733,7 +733,8 @@ struct PoRecompute {
 // f before the clamp; certain means it lies inside the range of the input
 // levels read so far, so the clamp against the whole column cannot change
 // it.  fixup() clamps the others once the column range is known.
-template <int NF>
+// NT: outputs written with non-temporal stores (streamed past the caches)
+template <int NF, bool NT = false>
 struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
   double* const* fp;
   int nf;
@@ -744,12 +745,21 @@ struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
   HYB_HD void emit(int kc, const double* v, bool certain, bool) {
 #pragma unroll
     for (int f = 0; f < NF; f++)
-      if (f < nf) fp[f][(int64_t)(kc - 1) * S] = v[f];
+      if (f < nf) put(fp[f] + (int64_t)(kc - 1) * S, v[f]);
     if (!certain) {
       fix |= 1ull << (kc - 1);
-    } else if (th3d) {
-      th3d[(int64_t)(kc - 1) * S] = sig(v[0], v[1]) - thbase;
+    } else if constexpr (NF >= 2) {  // th3d needs T and S in the same walk
+      if (th3d) put(th3d + (int64_t)(kc - 1) * S, sig(v[0], v[1]) - thbase);
     }
+  }
+  HYB_HD static void put(double* a, double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (NT) {
+      __builtin_nontemporal_store(x, a);
+      return;
+    }
+#endif
+    *a = x;
   }
   HYB_HD void fixup(const double* smin, const double* smax) {
     while (fix) {
@@ -767,7 +777,8 @@ struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
           v[f] = x;
         }
       }
-      if (th3d) th3d[(int64_t)(k - 1) * S] = sig(v[0], v[1]) - thbase;
+      if constexpr (NF >= 2)
+        if (th3d) th3d[(int64_t)(k - 1) * S] = sig(v[0], v[1]) - thbase;
     }
   }
 };
@@ -1422,6 +1433,60 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
 }
 
 // ---------------------------------------------------------------------
+// The p-column remap of T, S and the tracers: walks of NF fields at a
+// time (T and S first, with th3d), then one walk over the remaining
+// fields.  Every walk is instantiated for its exact field count, so no
+// field of a walk is switched off at run time.
+// ---------------------------------------------------------------------
+template <int METHOD, int NW, int PD, bool FAST, bool NT, class PO>
+HYB_HDI void remap_walk(const KParams& P, const View& V, int64_t c0, int n, int g0, int K,
+                        double zx, const double* pres, int kp, uint64_t lcm, PO& po) {
+  const int64_t S = V.NIJ;
+  double* fp[NW];
+#pragma unroll
+  for (int f = 0; f < NW; f++) {
+    const int g = g0 + f;
+    if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
+    else if (g == 1) fp[f] = V.saln + c0 + lev4(V, 1, n);
+    else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
+  }
+  double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
+  PiArray pi{pres, S};
+  FieldSink<NW, NT> sink{fp, NW, th, P.thbase, S};
+  walk_core<METHOD, NW, PD, FAST>(K, zx, pi, po, sink, fp, NW, kp, lcm, S);
+}
+
+// the last walk, over r < NF fields (r = 0: none)
+template <int METHOD, int R, int PD, bool FAST, bool NT, class PO>
+HYB_HDI void remap_rest(const KParams& P, const View& V, int64_t c0, int n, int g0, int r, int K,
+                        double zx, const double* pres, int kp, uint64_t lcm, PO& po) {
+  if constexpr (R >= 1) {
+    if (r == R) remap_walk<METHOD, R, PD, FAST, NT>(P, V, c0, n, g0, K, zx, pres, kp, lcm, po);
+    else remap_rest<METHOD, R - 1, PD, FAST, NT>(P, V, c0, n, g0, r, K, zx, pres, kp, lcm, po);
+  }
+}
+
+template <int METHOD, int NF, int PD, bool FAST, bool NT, class PO>
+HYB_HDI void remap_fields(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
+                          PO& po) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  const int kpw = W.kp[c0];
+  if (kpw & kUnsafe) return;
+  const int kp = kpw & 0xffff;
+  const uint64_t lcm = W.lcm[c0];
+  const double* pres = W.pres + c0;
+  const double zx = pres[(int64_t)K * S];
+  const int nfields = 2 + P.ntracr;
+  int g0 = 0;
+  for (; g0 + NF <= nfields; g0 += NF)
+    remap_walk<METHOD, NF, PD, FAST, NT>(P, V, c0, n, g0, K, zx, pres, kp, lcm, po);
+  remap_rest<METHOD, NF - 1, PD, FAST, NT>(P, V, c0, n, g0, nfields - g0, K, zx, pres, kp, lcm,
+                                           po);
+}
+
+// ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
 // walk of walk_core with FAST set.  The reconstruction (coef_layer) runs
 // in float with hardware reciprocals; the piece integrals, the output sums
@@ -1435,64 +1500,14 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
 // ---------------------------------------------------------------------
 template <int METHOD, int NF, int PD, class PO>
 HYB_HDI void remap_mixed(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
-                        PO& po) {
-  const int K = P.kk;
-  const int64_t c0 = ij_of(V, i, j);
-  const int64_t S = V.NIJ;
-  const int kpw = W.kp[c0];
-  if (kpw & kUnsafe) return;
-  const int kp = kpw & 0xffff;
-  const uint64_t lcm = W.lcm[c0];
-  const double* pres = W.pres + c0;
-  const double zx = pres[(int64_t)K * S];
-  const int nfields = 2 + P.ntracr;
-  for (int g0 = 0; g0 < nfields; g0 += NF) {
-    const int nf = (nfields - g0 < NF) ? nfields - g0 : NF;
-    double* fp[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      const int g = g0 + f;
-      if (f >= nf) fp[f] = nullptr;
-      else if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
-      else if (g == 1) fp[f] = V.saln + c0 + lev4(V, 1, n);
-      else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
-    }
-    double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
-    PiArray pi{pres, S};
-    FieldSink<NF> sink{fp, nf, th, P.thbase, S};
-    walk_core<METHOD, NF, PD, true>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
-  }
+                         PO& po) {
+  remap_fields<METHOD, NF, PD, true, false>(P, V, W, i, j, n, po);
 }
 
-template <int METHOD, int NF, int PD, class PO>
+template <int METHOD, int NF, int PD, class PO, bool NT = false>
 HYB_HDI void remap_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int n,
-                         PO& po) {
-  const int K = P.kk;
-  const int64_t c0 = ij_of(V, i, j);
-  const int64_t S = V.NIJ;
-  const int kpw = W.kp[c0];
-  if (kpw & kUnsafe) return;
-  const int kp = kpw & 0xffff;
-  const uint64_t lcm = W.lcm[c0];
-  const double* pres = W.pres + c0;
-  const double zx = pres[(int64_t)K * S];
-  const int nfields = 2 + P.ntracr;
-  for (int g0 = 0; g0 < nfields; g0 += NF) {
-    const int nf = (nfields - g0 < NF) ? nfields - g0 : NF;
-    double* fp[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      const int g = g0 + f;
-      if (f >= nf) fp[f] = nullptr;
-      else if (g == 0) fp[f] = V.temp + c0 + lev4(V, 1, n);
-      else if (g == 1) fp[f] = V.saln + c0 + lev4(V, 1, n);
-      else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
-    }
-    double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
-    PiArray pi{pres, S};
-    FieldSink<NF> sink{fp, nf, th, P.thbase, S};
-    walk_core<METHOD, NF, PD>(K, zx, pi, po, sink, fp, nf, kp, lcm, S);
-  }
+                          PO& po) {
+  remap_fields<METHOD, NF, PD, false, NT>(P, V, W, i, j, n, po);
 }
 
 // ---------------------------------------------------------------------

@@ -185,7 +185,8 @@ int hycom_hybgen_set_kernel_path(int path);
  * (6,0) (6,2) (3,0) (3,2) (2,2) (2,4), 6..13 the same walk with the new
  * interfaces read from p instead of LDS: (3,0) (3,0) (2,0) (2,2) (2,0)
  * (3,2) (3,2) (2,2) at 1, 3, 3, 3, 4, 1, 3, 4 waves per SIMD asked of the
- * compiler (6..13); knob 1: levels loaded ahead by the
+ * compiler (6..13), 14 = (3,4) and 15 = (3,2) with non-temporal output
+ * stores, both staged in LDS; knob 1: levels loaded ahead by the
  * u/v walk, 0 or 2; knob 2: the u/v walk's new thicknesses kept in LDS (0)
  * or formed again from p as the walk needs them (1, no LDS).  Process-wide. */
 int hycom_hybgen_tune(int knob, int value);

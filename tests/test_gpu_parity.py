@@ -208,7 +208,7 @@ def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
 
 
-@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 11, 12, 13, 0, 1, 2, 4, 5, 3])
+@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 0, 1, 2, 4, 5, 3])
 def test_streaming_remap_variants(gpu, variant):
     """Every p-column remap variant (fields per walk, prefetch, new
     interfaces from LDS or read ahead from p, occupancy target) gives the
