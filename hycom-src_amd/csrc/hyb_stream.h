@@ -670,7 +670,7 @@ struct PoAhead {
   int64_t S;
   int k1;           // kk+1, the deepest level
   int k = 1;        // level of cur
-  double prv = 0.0, cur = 0.0, nxt = 0.0;  // levels k-1, k, k+1
+  double cur = 0.0, nxt = 0.0;
   HYB_HD PoAhead(const double* p_, int64_t S_, int kk) : p(p_), S(S_), k1(kk + 1) {
     cur = p[0];
     nxt = p[S];
@@ -678,12 +678,10 @@ struct PoAhead {
   HYB_HD double operator()(int kq) {
     if (kq < k) {
       k = kq;
-      prv = (k >= 2) ? p[(int64_t)(k - 2) * S] : 0.0;
       cur = p[(int64_t)(k - 1) * S];
       nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
     }
     while (k < kq) {
-      prv = cur;
       cur = nxt;
       k++;
       nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
@@ -692,6 +690,44 @@ struct PoAhead {
   }
 };
 
+// The same prsf(k) without the LDS column: the new thicknesses are formed
+// again from p at the two neighbouring p-points as the walk asks for them
+// (the operations of the first pass of uv_stream, so the same bits).
+struct PoRecompute {
+  const double* p0;
+  const double* p1;
+  int64_t S;
+  double depthu;
+  int k1;                                // kk+1, the deepest level
+  double prv = 0.0, cur = 0.0, b = 0.0;  // prsf(k-1), prsf(k), min(depthu, p-mean at k)
+  double q0 = 0.0, q1 = 0.0;             // p at level k+1 of both points, loaded ahead
+  int k = 0;                             // 0 until the first request
+  HYB_HD PoRecompute(const double* a0, const double* a1, int64_t s, double du, int kk)
+      : p0(a0), p1(a1), S(s), depthu(du), k1(kk + 1) {}
+  HYB_HD double operator()(int kq) {
+    if (kq == 1 || k == 0) {
+      prv = 0.0;
+      cur = 0.0;
+      b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
+      k = 1;
+      q0 = p0[S];
+      q1 = p1[S];
+    }
+    while (k < kq) {
+      const double a = dmin(depthu, 0.5 * (q0 + q1));
+      const double d = dmax(0.0, a - b);
+      b = a;
+      prv = cur;
+      cur = cur + d;
+      k++;
+      if (k < k1) {
+        q0 = p0[(int64_t)k * S];
+        q1 = p1[(int64_t)k * S];
+      }
+    }
+    return cur;
+  }
+};
 
 // Output sinks.  emit(kc, v, certain, thick): v[f] is output kc of field
 // f before the clamp; certain means it lies inside the range of the input
@@ -1526,10 +1562,8 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
   PiSum pi{dpu, S};
   double* fp[1] = {uu};
   if (RECOMP) {
-    // prsf(k) as the first pass left it in pu (the same sums, so the
-    // same bits), read one level ahead
-    PoAhead po(pu, S, K);
-    UvSink<PoAhead> sink{uu, S, ub, pr, &po};
+    PoRecompute po(p0, p1, S, depthu, K);
+    UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
     walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
 #pragma unroll 2
