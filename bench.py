@@ -83,6 +83,22 @@ def cpu_baseline(workload, hybmap, rows, seed):
     times.sort()
     t = times[len(times) // 2]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    # the same reference on one thread, on the first rows // 8 rows of the band
+    one = None
+    lib = ol.ref(False)
+    if hasattr(lib, "omp_set_num_threads") and hasattr(lib, "omp_get_max_threads"):
+        nmax = lib.omp_get_max_threads()
+        P1, w1 = abi.workload_params(workload, hybmap=hybmap, jdm=max(rows // 8, 1))
+        d1 = abi.HostDomain(P1).synth(abi.make_synth(P1, w1, seed=seed, itdm=itdm, jtdm=jtdm,
+                                                     i0=0, j0=j0))
+        sea1 = int((d1.a["ip"][P1.nbdy:P1.nbdy + P1.ii, P1.nbdy:P1.nbdy + P1.jj] > 0).sum())
+        lib.omp_set_num_threads(1)
+        try:
+            t1 = sorted(ol.run_ref(d1.copy()) for _ in range(3))[1]
+        finally:
+            lib.omp_set_num_threads(nmax)
+        one = {"value": sea1 * P1.kdm / t1, "rows": P1.jdm, "sea_columns": sea1,
+               "seconds_per_call": t1}
     return {
         "value": sea * P.kdm / t,
         "unit": "column-layers/s",
@@ -91,6 +107,7 @@ def cpu_baseline(workload, hybmap, rows, seed):
         "sample": (f"HYCOM-src hybgen (flang -O2, OpenMP over j) on rows {j0 + 1}-{j0 + rows} "
                    f"of the {itdm}x{jtdm}x{P.kdm} field ({sea} sea columns), median of 3 "
                    f"calls of {t:.3f} s"),
+        "one_thread": one,
     }
 
 
@@ -121,19 +138,44 @@ def mixed_leg(lib, dd, S, sea, args, stream):
     Pm, _ = abi.workload_params(args.workload, hybmap=args.hybmap, idm=600, jdm=400)
     Pm.precision = abi.PREC_MIXED
     b = abi.DeviceDomain(Pm).synth(Sw)
+    torch.cuda.synchronize()
+    h0 = a.download()
     # both windows in one batched call (its kernels are the batch ones, so
     # the full-domain kernels' traces hold the full-domain calls only)
     abi.Batch([a, b]).hybgen(m=1, n=2)
     torch.cuda.synchronize()
     ha, hb = a.download(), b.download()
-    err = {}
+    import numpy as np
+    nb = Pw.nbdy
+    sea_w = ha.a["ip"][nb:nb + Pw.ii, nb:nb + Pw.jj] > 0
+    err, rms = {}, {}
     for f in ("temp", "saln", "th3d", "tracer", "u", "v"):
         x, y = ha.a[f], hb.a[f]
-        err[f] = float(abs(x - y).max() / max(abs(x).max(), 1e-300))
+        scale = max(float(abs(x).max()), 1e-300)
+        err[f] = float(abs(x - y).max() / scale)
+        rms[f] = float(np.sqrt(np.mean((x - y) ** 2)) / scale)
+    # column integrals sum_k dp*x at level n over sea columns: the change
+    # across the call (the remap's conservation residual, FP64 and mixed)
+    # and the mixed integral against the FP64 one
+    cons = {}
+
+    def colint(h, f, t=None):
+        x = h.a[f][..., 1] if t is None else h.a[f][..., 1, t]
+        return (h.a["dp"][..., 1] * x).sum(axis=2)[nb:nb + Pw.ii, nb:nb + Pw.jj][sea_w]
+
+    fields = [("temp", None), ("saln", None)] + [("tracer", t) for t in range(Pw.ntracr)]
+    for f, t in fields:
+        i0, i64, imx = colint(h0, f, t), colint(ha, f, t), colint(hb, f, t)
+        den = np.maximum(abs(i0), 1e-300)
+        key = f if t is None else f"tracer{t + 1}"
+        cons[key] = {"fp64_vs_input": float((abs(i64 - i0) / den).max()),
+                     "mixed_vs_input": float((abs(imx - i0) / den).max()),
+                     "mixed_vs_fp64": float((abs(imx - i64) / den).max())}
     same = all((ha.a[f] == hb.a[f]).all() for f in ("p", "dp", "dpu", "dpv", "pu", "pv"))
     return {"value": sea * P.kdm / t, "unit": "column-layers/s", "ms_per_step": 1e3 * t,
             "dtype": "f32 reconstruction, f64 interfaces and sums",
             "speedup_vs_fp64": None, "max_rel_error_vs_fp64": err,
+            "rms_rel_error_vs_fp64": rms, "column_integral_residual": cons,
             "interfaces_bitwise_fp64": bool(same),
             "error_window": "600x400 centre window of the same field"}
 
@@ -291,7 +333,7 @@ def serve_leg(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="glbb", choices=sorted(abi.WORKLOADS))
     ap.add_argument("--hybmap", type=int, default=2)
@@ -300,6 +342,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (HYB_PREC_MIXED) leg")
+    ap.add_argument("--no-weno", action="store_true", help="skip the WENO-remapper leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched ragged domains leg")
     ap.add_argument("--batch-domains", type=int, default=64)
@@ -387,6 +430,26 @@ def main():
     p50 = srt[len(srt) // 2] * 1e3
     p99 = srt[min(len(srt) - 1, int(round(0.99 * (len(srt) - 1))))] * 1e3
 
+    weno = None
+    if not args.no_weno and args.hybmap != 3:
+        # the same field and steps with the WENO remapper (hybmap 3)
+        P.hybmap = 3
+        for _ in range(args.warmup):
+            dd.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        tw = []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            dd.hybgen(m=1, n=2, stream=stream)
+            torch.cuda.synchronize()
+            tw.append(time.perf_counter() - ts)
+        P.hybmap = args.hybmap
+        tw.sort()
+        weno = {"value": sea * P.kdm * len(tw) / sum(tw), "unit": "column-layers/s",
+                "ms_per_step": 1e3 * sum(tw) / len(tw),
+                "ms_p50": 1e3 * tw[len(tw) // 2],
+                "ms_p99": 1e3 * tw[min(len(tw) - 1, int(round(0.99 * (len(tw) - 1))))]}
+        print(f"[bench] weno {weno['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
@@ -451,6 +514,7 @@ def main():
         "call_bytes_per_sea_column": bytes_per_column_call(P.kdm, P.ntracr),
         "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
         "cpu_baseline": cpu,
+        "weno": weno,
         "mixed": mixed,
         "batch": batch,
         "serve": serve,
