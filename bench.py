@@ -159,15 +159,15 @@ def mixed_leg(lib, dd, S, sea, args, stream):
     # and the mixed integral against the FP64 one
     cons = {}
 
-    def colint(h, f, t=None):
-        x = h.a[f][..., 1] if t is None else h.a[f][..., 1, t]
+    def colint(h, f, tr=None):
+        x = h.a[f][..., 1] if tr is None else h.a[f][..., 1, tr]
         return (h.a["dp"][..., 1] * x).sum(axis=2)[nb:nb + Pw.ii, nb:nb + Pw.jj][sea_w]
 
-    fields = [("temp", None), ("saln", None)] + [("tracer", t) for t in range(Pw.ntracr)]
-    for f, t in fields:
-        i0, i64, imx = colint(h0, f, t), colint(ha, f, t), colint(hb, f, t)
+    fields = [("temp", None), ("saln", None)] + [("tracer", q) for q in range(Pw.ntracr)]
+    for f, tr in fields:
+        i0, i64, imx = colint(h0, f, tr), colint(ha, f, tr), colint(hb, f, tr)
         den = np.maximum(abs(i0), 1e-300)
-        key = f if t is None else f"tracer{t + 1}"
+        key = f if tr is None else f"tracer{tr + 1}"
         cons[key] = {"fp64_vs_input": float((abs(i64 - i0) / den).max()),
                      "mixed_vs_input": float((abs(imx - i0) / den).max()),
                      "mixed_vs_fp64": float((abs(imx - i64) / den).max())}

@@ -761,7 +761,10 @@ struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
 #endif
     *a = x;
   }
-  HYB_HD void fixup(const double* smin, const double* smax) {
+  // clamp to [smin, smax] the outputs outside [lo, hi] (lo = smin and
+  // hi = smax except in the mixed form, see walk_core)
+  HYB_HD void fixup(const double* lo, const double* hi, const double* smin,
+                    const double* smax) {
     while (fix) {
       const int k = __builtin_ctzll(fix) + 1;
       fix &= fix - 1;
@@ -771,8 +774,8 @@ struct FieldSink {  // T, S, tracers; th3d from T and S (hybgen.F90:1549-1569)
         v[f] = 0.0;
         if (f < nf) {
           double x = fp[f][(int64_t)(k - 1) * S];
-          x = dmax(smin[f], x);
-          x = dmin(smax[f], x);
+          x = (x < lo[f]) ? smin[f] : x;
+          x = (x > hi[f]) ? smax[f] : x;
           fp[f][(int64_t)(k - 1) * S] = x;
           v[f] = x;
         }
@@ -798,13 +801,14 @@ struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
       uu[(int64_t)(kc - 1) * S] = -ub;
     }
   }
-  HYB_HD void fixup(const double* smin, const double* smax) {
+  HYB_HD void fixup(const double* lo, const double* hi, const double* smin,
+                    const double* smax) {
     while (fix) {
       const int k = __builtin_ctzll(fix) + 1;
       fix &= fix - 1;
       double x = uu[(int64_t)(k - 1) * S];
-      x = dmax(smin[0], x);
-      x = dmin(smax[0], x);
+      x = (x < lo[0]) ? smin[0] : x;
+      x = (x > hi[0]) ? smax[0] : x;
       uu[(int64_t)(k - 1) * S] = x;
     }
   }
@@ -1135,6 +1139,14 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
   int lb_act = 0;
   double zt_act = 0.0, zb_act = 0.0;
 
+  // The clamp to the column range (e.g. 2836-2837).  The mixed form lets
+  // values within a slack of 2^-20 of the range's magnitude stand: its
+  // float reconstruction can overshoot an edge that the FP64 one meets
+  // exactly, and clamping those by rounding error would break the
+  // conservation that the remap otherwise keeps.
+  auto slack = [&](int f) -> double {
+    return FAST ? 0x1p-20 * dmax(dabs(smin[f]), dabs(smax[f])) : 0.0;
+  };
   // output kc complete: v[f] before the clamp
   auto emit = [&](const double* v) {
     bool certain = true;
@@ -1142,7 +1154,12 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     for (int f = 0; f < NF; f++)
       if (f < nf) {
         prev[f] = v[f];
-        if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
+        if (FAST) {
+          const double sl = slack(f);
+          if (!(v[f] >= smin[f] - sl && v[f] <= smax[f] + sl)) certain = false;
+        } else if (!(v[f] >= smin[f] && v[f] <= smax[f])) {
+          certain = false;
+        }
       }
     sink.emit(kc, v, certain, (thick >> (kc - 1)) & 1ull);
     kc = kc + 1;
@@ -1179,6 +1196,21 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
       coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lc_l, thf, 1.0e-8f, fm, f0, fq,
                                     gmax(fm, thf), gmax(f0, thf), gmax(fq, thf), gmax(fq2, thf),
                                     sf0, sf1, sf2, sf3, cst, g1, g2, g3);
+      // The limited edge values lie between the neighbouring layer means
+      // in exact arithmetic, and where a limiter sets one to a neighbour's
+      // mean it is that mean.  Their float rounding can step outside, and
+      // a reconstruction outside the column's range is then clamped (which
+      // is not conservative).  So the edges come back to FP64 as the
+      // neighbour's FP64 mean where they equal its float value, and held
+      // to [min, max] of s(l-1), s(l), s(l+1) otherwise.
+      auto held = [&](int f, float e) -> double {
+        if (e == sf1[f]) return s1[f];
+        if (e == sf0[f]) return s0[f];
+        if (e == sf2[f]) return s2[f];
+        const double lo = dmin(dmin(s0[f], s1[f]), s2[f]);
+        const double hi = dmax(dmax(s0[f], s1[f]), s2[f]);
+        return dmin(dmax((double)e, lo), hi);
+      };
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         c1l[f] = g1[f];
@@ -1186,10 +1218,20 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
         c3l[f] = g3[f];
         if (METHOD == 2) {
           if (g2[f] == 0.0f && g3[f] == 0.0f) {
-            if (g1[f] == sf1[f]) c1l[f] = s1[f];  // flat: exactly s(l)
+            // flat: exactly s(l).  Edge values that coincide in float
+            // without the limiter setting them (rare in FP64, not in
+            // float for a nearly uniform field) are taken as flat too,
+            // so every layer keeps its mean
+            c1l[f] = s1[f];
           } else {
-            c3l[f] = 6.0 * s1[f] - 3.0 * (c1l[f] + (c1l[f] + c2l[f]));
+            const double al = held(f, g1[f]), ar = held(f, g1[f] + g2[f]);
+            c1l[f] = al;
+            c2l[f] = ar - al;
+            c3l[f] = 6.0 * s1[f] - 3.0 * (al + ar);
           }
+        } else if (METHOD == 3) {
+          c1l[f] = held(f, g1[f]);
+          c2l[f] = held(f, g2[f]);
         }
       }
     }
@@ -1429,7 +1471,13 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     }
     xl++;
   }
-  sink.fixup(smin, smax);
+  double lo[NF], hi[NF];
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    lo[f] = FAST ? smin[f] - slack(f) : smin[f];
+    hi[f] = FAST ? smax[f] + slack(f) : smax[f];
+  }
+  sink.fixup(lo, hi, smin, smax);
 }
 
 // ---------------------------------------------------------------------
