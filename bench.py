@@ -360,6 +360,8 @@ def main():
                          "the library default)")
     ap.add_argument("--uv-prefetch", type=int, default=None,
                     help="streaming u/v prefetch depth 0 or 2 (knob 1)")
+    ap.add_argument("--regrid-ahead", type=int, default=None,
+                    help="regrid window loaded one iteration ahead (1) or not (0; knob 3)")
     ap.add_argument("--uv-recompute", type=int, default=None,
                     help="u/v new thicknesses from LDS (0) or formed again from p (1; knob 2)")
     args = ap.parse_args()
@@ -378,6 +380,8 @@ def main():
         abi.check(lib.hycom_hybgen_tune(1, args.uv_prefetch))
     if args.uv_recompute is not None:
         abi.check(lib.hycom_hybgen_tune(2, args.uv_recompute))
+    if args.regrid_ahead is not None:
+        abi.check(lib.hycom_hybgen_tune(3, args.regrid_ahead))
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     P, w = abi.workload_params(args.workload, hybmap=args.hybmap)

@@ -696,6 +696,11 @@ int hycom_hybgen_tune(int knob, int value) {
     g_tune.uv_recompute = value;
     return 0;
   }
+  if (knob == 3) {
+    if (value != 0 && value != 1) return fail(HYB_EINVAL, "regrid read-ahead %d: 0 or 1", value);
+    g_tune.regrid_ahead = value;
+    return 0;
+  }
   return fail(HYB_EINVAL, "tuning knob %d", knob);
 }
 

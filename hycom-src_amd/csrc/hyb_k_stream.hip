@@ -13,6 +13,17 @@ __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch 
   regrid_stream(P, V, W, i, j, n);
 }
 
+// the regrid with its window loaded one iteration ahead, held to 3 waves
+// per SIMD
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_regrid_a(KParams P, View V, Scratch W, int n) {
+  int j = blockIdx.y + 1;
+  int i = i_aligned0(V, j) + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i > P.ii) return;
+  if (V.ip[ij_of(V, i, j)] == 0) return;
+  regrid_stream<true>(P, V, W, i, j, n);
+}
+
 template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>
 __global__ void __launch_bounds__(kWave) k_uv_s(KParams P, View V, Scratch W, int nl) {
   extern __shared__ double lds[];
@@ -74,7 +85,10 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
     PhaseMark pm(T, s, 0);
     {
       PhaseMark pk(T, s, 6);
-      hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
+      if (tune.regrid_ahead)
+        hipLaunchKernelGGL(k_regrid_a, g2, blk, 0, s, P, V, W, n);
+      else
+        hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
     }
     {
       PhaseMark pk(T, s, 7);

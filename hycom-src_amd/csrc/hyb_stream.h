@@ -76,8 +76,11 @@ HYB_HD void shl(double (&a)[W], double v) {
 // ---------------------------------------------------------------------
 // regrid of one p-column (hybgenaj up to the remap)
 // ---------------------------------------------------------------------
+// AHEAD: the pass-3 window's next levels (pres, th3d, theta) are loaded
+// one layer iteration before the slide takes them in (more registers)
+template <bool AHEAD = false>
 HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, int i, int j,
-                          int n) {
+                           int n) {
   const int K = P.kk;
   const int64_t c0 = ij_of(V, i, j);
   const int64_t S = V.NIJ;
@@ -394,6 +397,17 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     }
   };
 
+  // with AHEAD: pres(k+4), th3d(k+3), theta(k+3) for the slide of
+  // iteration k, loaded during iteration k-1 (indices clamped into the
+  // column; values beyond it are not used)
+  auto cl1 = [&](int x) -> int64_t { return (int64_t)((x <= K + 1 ? x : K + 1) - 1) * S; };
+  auto clk = [&](int x) -> int64_t { return (int64_t)((x <= K ? x : K) - 1) * S; };
+  double nr4 = 0.0, nth3 = 0.0, nta3 = 0.0;
+  if (AHEAD) {
+    nr4 = pres[cl1(6)];
+    nth3 = th3d[clk(5)];
+    nta3 = theta[clk(5)];
+  }
   double q, p_hat, p_hat0, p_hat2, p_hat3;
 #pragma unroll 1
   for (int k = 2; k <= K; k++) {
@@ -577,10 +591,20 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     // p(k-2) is final now
     if (k >= 3) consume(k - 2, pw[1], pchk);
     // slide: level k+4 enters pw, k+3 enters tw/thw/lcm, k+2 enters dw/qw
-    const double r4 = raw_at(k + 4);
+    double r4, th3, ta3;
+    if (AHEAD) {
+      r4 = (k + 4 <= K + 1) ? nr4 : 0.0;
+      th3 = (k + 3 <= K) ? (k + 3 > kp ? th_kp : nth3) : 0.0;
+      ta3 = (k + 3 <= K) ? nta3 : 0.0;
+      nr4 = pres[cl1(k + 5)];
+      nth3 = th3d[clk(k + 4)];
+      nta3 = theta[clk(k + 4)];
+    } else {
+      r4 = raw_at(k + 4);
+      th3 = (k + 3 <= K) ? th3d_eff(k + 3) : 0.0;
+      ta3 = (k + 3 <= K) ? FK(theta, k + 3) : 0.0;
+    }
     shl(pw, (k + 4 <= K + 1) ? with_floor(k + 4, r4) : 0.0);
-    const double th3 = (k + 3 <= K) ? th3d_eff(k + 3) : 0.0;
-    const double ta3 = (k + 3 <= K) ? FK(theta, k + 3) : 0.0;
     if (k + 3 <= K) lcm_set(k + 3, rw[2], r4, th3, ta3);
     shl(tw, th3);
     shl(thw, ta3);
@@ -1140,10 +1164,10 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
   double zt_act = 0.0, zb_act = 0.0;
 
   // The clamp to the column range (e.g. 2836-2837).  The mixed form lets
-  // values within a slack of 2^-20 of the range's magnitude stand: its
-  // float reconstruction can overshoot an edge that the FP64 one meets
-  // exactly, and clamping those by rounding error would break the
-  // conservation that the remap otherwise keeps.
+  // values within a slack of 2^-20 of the range's magnitude stand (in the
+  // final fixup): its float reconstruction can overshoot an edge that the
+  // FP64 one meets exactly, and clamping those by rounding error would
+  // break the conservation that the remap otherwise keeps.
   auto slack = [&](int f) -> double {
     return FAST ? 0x1p-20 * dmax(dabs(smin[f]), dabs(smax[f])) : 0.0;
   };
@@ -1154,12 +1178,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     for (int f = 0; f < NF; f++)
       if (f < nf) {
         prev[f] = v[f];
-        if (FAST) {
-          const double sl = slack(f);
-          if (!(v[f] >= smin[f] - sl && v[f] <= smax[f] + sl)) certain = false;
-        } else if (!(v[f] >= smin[f] && v[f] <= smax[f])) {
-          certain = false;
-        }
+        if (!(v[f] >= smin[f] && v[f] <= smax[f])) certain = false;
       }
     sink.emit(kc, v, certain, (thick >> (kc - 1)) & 1ull);
     kc = kc + 1;
@@ -1196,21 +1215,6 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
       coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lc_l, thf, 1.0e-8f, fm, f0, fq,
                                     gmax(fm, thf), gmax(f0, thf), gmax(fq, thf), gmax(fq2, thf),
                                     sf0, sf1, sf2, sf3, cst, g1, g2, g3);
-      // The limited edge values lie between the neighbouring layer means
-      // in exact arithmetic, and where a limiter sets one to a neighbour's
-      // mean it is that mean.  Their float rounding can step outside, and
-      // a reconstruction outside the column's range is then clamped (which
-      // is not conservative).  So the edges come back to FP64 as the
-      // neighbour's FP64 mean where they equal its float value, and held
-      // to [min, max] of s(l-1), s(l), s(l+1) otherwise.
-      auto held = [&](int f, float e) -> double {
-        if (e == sf1[f]) return s1[f];
-        if (e == sf0[f]) return s0[f];
-        if (e == sf2[f]) return s2[f];
-        const double lo = dmin(dmin(s0[f], s1[f]), s2[f]);
-        const double hi = dmax(dmax(s0[f], s1[f]), s2[f]);
-        return dmin(dmax((double)e, lo), hi);
-      };
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         c1l[f] = g1[f];
@@ -1224,14 +1228,8 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
             // so every layer keeps its mean
             c1l[f] = s1[f];
           } else {
-            const double al = held(f, g1[f]), ar = held(f, g1[f] + g2[f]);
-            c1l[f] = al;
-            c2l[f] = ar - al;
-            c3l[f] = 6.0 * s1[f] - 3.0 * (al + ar);
+            c3l[f] = 6.0 * s1[f] - 3.0 * (c1l[f] + (c1l[f] + c2l[f]));
           }
-        } else if (METHOD == 3) {
-          c1l[f] = held(f, g1[f]);
-          c2l[f] = held(f, g2[f]);
         }
       }
     }

@@ -188,7 +188,8 @@ int hycom_hybgen_set_kernel_path(int path);
  * compiler (6..13), 14 = (3,4) and 15 = (3,2) with non-temporal output
  * stores, both staged in LDS; knob 1: levels loaded ahead by the
  * u/v walk, 0 or 2; knob 2: the u/v walk's new thicknesses kept in LDS (0)
- * or formed again from p as the walk needs them (1, no LDS).  Process-wide. */
+ * or formed again from p as the walk needs them (1, no LDS); knob 3: the
+ * regrid's window loaded one iteration ahead (1) or not (0).  Process-wide. */
 int hycom_hybgen_tune(int knob, int value);
 
 /* Measurement hooks (no reference counterpart; the reference's only timer

@@ -162,7 +162,7 @@ extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, i
 #include "hyb_stream.h"
 
 static int64_t g_unsafe_cols = 0;
-static int g_uv_recompute = 0;  // 1: remap and u/v walks without the LDS column
+static int g_uv_recompute = 0;  // 1: the read-ahead regrid, remap and u/v variants
 static int64_t g_uv_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
@@ -182,7 +182,10 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
 #pragma omp parallel for schedule(dynamic, 1)
   for (int j = 1; j <= jj; j++)
     for (int i = 1; i <= ii; i++)
-      if (V.ip[ij_of(V, i, j)]) regrid_stream(P, V, W, i, j, n);
+      if (V.ip[ij_of(V, i, j)]) {
+        if (g_uv_recompute) regrid_stream<true>(P, V, W, i, j, n);
+        else regrid_stream(P, V, W, i, j, n);
+      }
   int64_t unsafe = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : unsafe)
   for (int j = 1; j <= jj; j++) {
