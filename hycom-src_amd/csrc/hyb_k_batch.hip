@@ -37,14 +37,15 @@ __device__ __forceinline__ RowPos row_pos(const BatchDevice& B, const BatchGroup
 }
 
 template <int BW>
-__global__ void __launch_bounds__(BW) kb_regrid(BatchDevice B, BatchGroup G, int n) {
+__global__ void __launch_bounds__(BW) __attribute__((amdgpu_waves_per_eu(3)))
+kb_regrid(BatchDevice B, BatchGroup G, int n) {
   const RowPos r = row_pos(B, G, BW, blockIdx.x);
   const KParams& P = B.P[r.d];
   const View& V = B.V[r.d];
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, r.j)] == 0) return;
-  regrid_stream(P, V, B.W[r.d], i, r.j, n);
+  regrid_stream<true>(P, V, B.W[r.d], i, r.j, n);
 }
 
 // CW columns per workgroup; each lane's LDS column has stride CW doubles

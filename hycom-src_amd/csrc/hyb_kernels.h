@@ -41,7 +41,7 @@ struct StreamTuning {
   int remap_variant = 3;  // 3 fields per walk, 2 levels loaded ahead (fastest measured)
   int uv_prefetch = 2;
   int uv_recompute = 1;  // u/v new thicknesses formed again from p, no LDS (fastest measured)
-  int regrid_ahead = 0;  // 1: the regrid's window loaded one iteration ahead
+  int regrid_ahead = 1;  // the regrid's window loaded one iteration ahead (fastest measured)
 };
 // Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
 // plus uvf (2*NIJ int32).

@@ -231,9 +231,10 @@ def test_streaming_remap_variants(gpu, variant):
 
 
 @pytest.mark.parametrize("kdm,hybmap,extreme", [(41, 2, 1), (22, 3, 0), (64, 1, 1)])
-def test_regrid_read_ahead(gpu, kdm, hybmap, extreme):
-    """The regrid variant that loads its window one iteration ahead (tuning
-    knob 3) gives the oracle's bits."""
+def test_regrid_without_read_ahead(gpu, kdm, hybmap, extreme):
+    """The regrid variant that loads its window as it takes it in (tuning
+    knob 3 = 0; the default loads it one iteration ahead) gives the
+    oracle's bits."""
     dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
     P, w = _case("regrid-ahead", idm=90, jdm=57, kdm=kdm, hybmap=hybmap, nsigma=min(8, kdm),
                  **dx)
@@ -242,11 +243,11 @@ def test_regrid_read_ahead(gpu, kdm, hybmap, extreme):
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm + 11, extreme=extreme))
     want = _oracle_out(d0)
     dd = abi.DeviceDomain(P).upload(d0)
-    abi.check(lib.hycom_hybgen_tune(3, 1))
+    abi.check(lib.hycom_hybgen_tune(3, 0))
     try:
         dd.hybgen(m=1, n=2)
         got = dd.download()
     finally:
-        abi.check(lib.hycom_hybgen_tune(3, 0))
+        abi.check(lib.hycom_hybgen_tune(3, 1))
         abi.check(lib.hycom_hybgen_set_kernel_path(0))
-    _assert_same(want, got, f"regrid read-ahead kdm={kdm} hybmap={hybmap}")
+    _assert_same(want, got, f"regrid without read-ahead kdm={kdm} hybmap={hybmap}")
