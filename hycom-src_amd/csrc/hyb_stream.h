@@ -1380,22 +1380,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
           } else {
             lb = l + 3;
             double pc = pv4;  // pi(l+3)
-            bool found = false;
-            // the PD interfaces loaded ahead first (prq[q] is raw(l+3+q)),
-            // then the rest from memory, one dependent load each
-#pragma unroll
-            for (int q = 0; q < PD; q++) {
-              if (!found && lb < K) {
-                const double pn = pi.comb(pc, prq[q]);
-                if (!(pn < zb)) {
-                  found = true;
-                } else {
-                  lb++;
-                  pc = pn;
-                }
-              }
-            }
-            while (!found && lb < K) {
+            while (lb < K) {
               const double pn = pi.nextfrom(lb, pc);
               if (!(pn < zb)) break;
               lb++;
@@ -1427,23 +1412,13 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
 #pragma unroll
             for (int f = 0; f < NF; f++) {
               if (f < nf) {
-                // o = s(mid), with the fill; levels > l+2 are not written
-                // yet, and levels l+3..l+2+PD are already loaded (the
-                // stored value of level min(mid, kp), as below)
+                // o = s(mid), with the fill; levels > l+2 are not written yet
                 double o;
-                bool have = true;
                 if (mid == l) o = s1[f];
                 else if (mid == l + 1) o = s2[f];
                 else if (mid == l + 2) o = s3[f];
                 else if (kp <= l + 2) o = skp[f];
-                else have = false;
-#pragma unroll
-                for (int q = 0; q < PD; q++)
-                  if (!have && mid == l + 3 + q) {
-                    o = sq[f][q];
-                    have = true;
-                  }
-                if (!have) o = fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S];
+                else o = fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S];
                 ov[f] = o;
                 if (METHOD == 0) {
                   acc[f] = dpt * (s1[f] - o);
