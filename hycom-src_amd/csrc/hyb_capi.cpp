@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdint.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -261,6 +262,45 @@ int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
   HIPCHK(hipMalloc(&w.uvf, sizeof(int32_t) * 2 * nij));
   g_ws.push_back(w);
   *W = Scratch{w.pres, w.kp, w.lcm, w.uvf};
+  return 0;
+}
+
+// the optional NaN/Inf scan: one int32 per scan block, grown as needed
+std::mutex g_scan_mu;
+int32_t* g_scan = nullptr;
+size_t g_scan_n = 0;
+
+const char* scan_field_name(int f) {
+  static const char* names[] = {"dp", "temp", "saln", "th3d", "theta", "depths", "topiso",
+                                "u", "dpu", "depthu", "ubavg", "v", "dpv", "depthv", "vbavg"};
+  return f >= 0 && f < 15 ? names[f] : "tracer";
+}
+
+// HYB_ENONFINITE with the first bad point (row order) if an input read by
+// the call is NaN or Inf; waits for the stream
+int scan_finite(const KParams& K, const View& V, int n, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_scan_mu);
+  const size_t nb = (size_t)scan_blocks(K);
+  if (nb > g_scan_n) {
+    if (g_scan) (void)hipFree(g_scan);
+    g_scan = nullptr;
+    g_scan_n = 0;
+    HIPCHK(hipMalloc(&g_scan, sizeof(int32_t) * nb));
+    g_scan_n = nb;
+  }
+  HIPCHK(launch_scan_finite(K, V, n, g_scan, s));
+  std::vector<int32_t> h(nb);
+  HIPCHK(hipMemcpyAsync(h.data(), g_scan, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int nbx = (int)(nb / (size_t)K.jj);
+  for (size_t b = 0; b < nb; b++) {
+    if (h[b] == INT32_MAX) continue;
+    const int j = (int)(b / (size_t)nbx) + 1, i = h[b] / 128, f = h[b] % 128;
+    if (f >= 16)
+      return fail(HYB_ENONFINITE, "non-finite %s %d at i=%d j=%d", scan_field_name(f), f - 16, i,
+                  j);
+    return fail(HYB_ENONFINITE, "non-finite %s at i=%d j=%d", scan_field_name(f), i, j);
+  }
   return 0;
 }
 
@@ -574,6 +614,12 @@ int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream
     a = (const unsigned char*)&v;
     key.insert(key.end(), a, a + sizeof v);
   }
+  for (int d = 0; d < ndom; d++)
+    if (D[d].P->check_finite) {
+      const KParams K = kparams_of(D[d].P);
+      const View V = view_of(D[d].P, D[d].F);
+      if ((rc = scan_finite(K, V, n, (hipStream_t)stream))) return rc;
+    }
   std::lock_guard<std::mutex> lk(g_batch_mu);
   BatchPlan& b = g_plan;
   if (b.key != key || b.ndom != ndom) {
@@ -645,10 +691,10 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
     return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm and no HYBGEN_DENFIX");
   if (P->precision == HYB_PREC_MIXED && (g_kernel_path == 1 || g_kernel_path == 2))
     return fail(HYB_EUNSUPPORTED, "the mixed form runs on the streaming kernels only");
-  if (P->check_finite) return fail(HYB_EUNSUPPORTED, "finite scan not built yet");
   if ((rc = device_ready())) return rc;
   KParams K = kparams_of(P);
   View V = view_of(P, F);
+  if (P->check_finite && (rc = scan_finite(K, V, n, (hipStream_t)stream))) return rc;
   V.rec_fixlay = g_rec_fixlay;
   V.rec_kp = g_rec_kp;
   const int path = g_kernel_path;

@@ -1,5 +1,7 @@
 // hyb_k_misc.hip -- p halo, diagnostic tracers and the synthetic-input
 // generator.
+#include <stdint.h>
+
 #include "hyb_launch.h"
 #include "hyb_synth.h"
 
@@ -40,6 +42,73 @@ __global__ void k_diag_post(KParams P, View V, int n, int t, const double* src, 
     int64_t a = c0 + levt(V, k, n, t);
     V.tracer[a] = q * (src[c0 + lev4(V, k, n)] - V.tracer[a]);
   }
+}
+
+// Scan of the inputs of one call for NaN/Inf (hyb_params.check_finite):
+// sea p-points (dp, T, S, th3d, theta, tracers at level n; depths, topiso),
+// u-points (u, dpu at n; depthu, ubavg) and v-points alike.  Each block
+// writes the smallest key i*128 + field of its bad columns (INT32_MAX if
+// none) to out[block]; the host reads the first bad block in row order.
+__global__ void __launch_bounds__(kBlock) k_scan_finite(KParams P, View V, int n,
+                                                         int32_t* out) {
+  __shared__ int32_t red[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1;
+  int32_t key = INT32_MAX;
+  if (i <= P.ii) {
+    const int64_t c0 = ij_of(V, i, j);
+    int field = -1;
+    auto check = [&](const double* a, int id) {
+      if (field < 0 && !__builtin_isfinite(*a)) field = id;
+    };
+    if (V.ip[c0]) {
+      for (int k = 1; k <= P.kk; k++) {
+        const int64_t a = c0 + lev4(V, k, n);
+        check(V.dp + a, 0);
+        check(V.temp + a, 1);
+        check(V.saln + a, 2);
+        check(V.th3d + a, 3);
+        check(V.theta + c0 + lev(V, k), 4);
+        for (int t = 1; t <= P.ntracr; t++) check(V.tracer + c0 + levt(V, k, n, t), 16 + t);
+      }
+      check(V.depths + c0, 5);
+      check(V.topiso + c0, 6);
+    }
+    if (V.iu[c0]) {
+      for (int k = 1; k <= P.kk; k++) {
+        check(V.u + c0 + lev4(V, k, n), 7);
+        check(V.dpu + c0 + lev4(V, k, n), 8);
+      }
+      check(V.depthu + c0, 9);
+      check(V.ubavg + c0 + lev(V, n), 10);
+    }
+    if (V.iv[c0]) {
+      for (int k = 1; k <= P.kk; k++) {
+        check(V.v + c0 + lev4(V, k, n), 11);
+        check(V.dpv + c0 + lev4(V, k, n), 12);
+      }
+      check(V.depthv + c0, 13);
+      check(V.vbavg + c0 + lev(V, n), 14);
+    }
+    if (field >= 0) key = i * 128 + field;
+  }
+  red[threadIdx.x] = key;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && red[threadIdx.x + w] < red[threadIdx.x])
+      red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+int scan_blocks(const KParams& P) { return ((P.ii + kBlock - 1) / kBlock) * P.jj; }
+
+hipError_t launch_scan_finite(const KParams& P, const View& V, int n, int32_t* out,
+                              hipStream_t s) {
+  dim3 g((P.ii + kBlock - 1) / kBlock, P.jj);
+  hipLaunchKernelGGL(k_scan_finite, g, dim3(kBlock), 0, s, P, V, n, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_halo(const KParams& P, const View& V, hipStream_t s) {

@@ -82,6 +82,12 @@ struct BatchLaunch {
 // Enqueues one hybgen(m,n,.false.) step of every domain of the batch.
 hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipStream_t s);
 
+// NaN/Inf scan of a call's inputs: scan_blocks(P) int32 results in out
+// (hyb_k_misc.hip; the key of a bad column is i*128 + field).
+int scan_blocks(const KParams& P);
+hipError_t launch_scan_finite(const KParams& P, const View& V, int n, int32_t* out,
+                              hipStream_t s);
+
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,
                         int m, int n, hipStream_t s);

@@ -89,7 +89,12 @@ typedef struct hyb_params {
   int32_t itracr802;     /* tracer index or 0: temp change diagnostic        */
   int32_t itracr803;     /* tracer index or 0: saln change diagnostic        */
   int32_t precision;     /* HYB_PREC_FP64 or HYB_PREC_MIXED                  */
-  int32_t check_finite;  /* logical: scan sea-point inputs for NaN/Inf first */
+  int32_t check_finite;  /* logical: scan the inputs the call reads at sea p-,
+                            u- and v-points for NaN/Inf first; the first bad
+                            point in row order is reported with
+                            HYB_ENONFINITE and nothing is computed (the scan
+                            waits for the stream).  Not used by the request
+                            loop. */
   int32_t trcflg[HYB_TRMAX];
   double  thbase, qhybrlx, hybiso, hybthn, hybthk, thkbot, dp00i;
   double  dpns, dsns;    /* metres (geopar.F90:629-630)                      */

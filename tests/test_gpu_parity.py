@@ -251,3 +251,34 @@ def test_regrid_without_read_ahead(gpu, kdm, hybmap, extreme):
         abi.check(lib.hycom_hybgen_tune(3, 1))
         abi.check(lib.hycom_hybgen_set_kernel_path(0))
     _assert_same(want, got, f"regrid without read-ahead kdm={kdm} hybmap={hybmap}")
+
+
+def test_finite_scan(gpu):
+    """check_finite: a NaN or Inf in an input the call reads stops the call
+    with HYB_ENONFINITE naming the first bad point, before anything is
+    written; without it the call runs; a clean domain gives the same bits
+    with and without the scan."""
+    import torch
+    P, w = _case("finite-scan", idm=60, jdm=40, kdm=32, ntracr=2, trcflg=[0, 0])
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=3))
+    want = _oracle_out(d0)
+    P.check_finite = 1
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    torch.cuda.synchronize()
+    _assert_same(want, dd.download(), "clean domain with the scan")
+    nb = P.nbdy
+    ip = d0.a["ip"]
+    sea = np.argwhere(ip[nb:nb + P.ii, nb:nb + P.jj] > 0)
+    i0, j0 = (int(x) for x in sea[len(sea) // 2])
+    for field, extra in (("temp", ()), ("tracer", (1,))):
+        bad = d0.copy()
+        idx = (i0 + nb, j0 + nb, 5, 1) + extra
+        bad.a[field][idx] = np.nan if field == "temp" else np.inf
+        dd = abi.DeviceDomain(P).upload(bad)
+        with pytest.raises(abi.HybError) as e:
+            dd.hybgen(m=1, n=2)
+        assert e.value.code == abi.HYB_ENONFINITE
+        assert f"i={i0 + 1} j={j0 + 1}" in str(e.value), str(e.value)
+        torch.cuda.synchronize()
+        assert not ol.diff_domains(bad, dd.download()), "the scan must stop the call"
