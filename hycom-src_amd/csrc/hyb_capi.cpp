@@ -728,7 +728,7 @@ int hycom_hybgen_set_kernel_path(int path) {
 
 int hycom_hybgen_tune(int knob, int value) {
   if (knob == 0) {
-    if (value < 0 || value > 15) return fail(HYB_EINVAL, "remap variant %d outside 0..15", value);
+    if (value < 0 || value > 16) return fail(HYB_EINVAL, "remap variant %d outside 0..16", value);
     g_tune.remap_variant = value;
     return 0;
   }

@@ -11,7 +11,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SO = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
 FAMILIES = {"lane-arrays": "hostcheck_hybgen", "register-resident": "hostcheck_hybgen_fast",
             "streaming": "hostcheck_hybgen_stream",
-            "streaming-recompute": "hostcheck_hybgen_stream_rc"}
+            "streaming-recompute": "hostcheck_hybgen_stream_rc",
+            "streaming-tiled": "hostcheck_hybgen_stream_tiled"}
 _lib = None
 
 
@@ -24,6 +25,7 @@ def lib():
                                        ctypes.POINTER(abi.HybFields),
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
         h.hostcheck_stream_unsafe_columns.restype = ctypes.c_int64
+        h.hostcheck_tile_column.restype = None
         h.hostcheck_stream_uv_fallback_points.restype = ctypes.c_int64
         _lib = h
     return _lib
@@ -33,7 +35,7 @@ def covers(family, P):
     ok = P.ldenfix == 0 and P.nhybrd == P.kdm
     if family == "register-resident":
         return ok and P.kdm in (22, 32, 41)
-    if family in ("streaming", "streaming-recompute"):
+    if family in ("streaming", "streaming-recompute", "streaming-tiled"):
         return ok
     return True
 

@@ -208,7 +208,28 @@ def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
 
 
-@pytest.mark.parametrize("variant", [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 0, 1, 2, 4, 5, 3])
+@pytest.mark.parametrize("kdm", [4, 9, 22, 31, 41, 47, 63])
+def test_tiled_remap(gpu, kdm):
+    """The tiled PPM remap (remap variant 16) with 2, 3 and 4 layers per
+    thread gives the oracle's bits, fallback columns included."""
+    P, w = _case("tiled-remap", idm=90, jdm=57, kdm=kdm, hybmap=2, ntracr=3,
+                 trcflg=[2, 0, 0], nsigma=min(8, kdm), dx00=30.0, dx00f=1.1, dx00x=300.0)
+    _select(3, P)
+    lib = abi.load()
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm + 5, extreme=kdm % 2))
+    want = _oracle_out(d0)
+    dd = abi.DeviceDomain(P).upload(d0)
+    abi.check(lib.hycom_hybgen_tune(0, 16))
+    try:
+        dd.hybgen(m=1, n=2)
+        got = dd.download()
+    finally:
+        abi.check(lib.hycom_hybgen_tune(0, 3))
+        abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    _assert_same(want, got, f"tiled remap kdm={kdm}")
+
+
+@pytest.mark.parametrize("variant", [16, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 0, 1, 2, 4, 5, 3])
 def test_streaming_remap_variants(gpu, variant):
     """Every p-column remap variant (fields per walk, prefetch, new
     interfaces from LDS or read ahead from p, occupancy target) gives the

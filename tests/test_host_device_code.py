@@ -52,6 +52,23 @@ def test_device_code_on_host_matches_oracle(family, kdms, draw):
     assert not df, f"{family} {over} raflag={raflag} m={m} n={n}: {df}"
 
 
+@pytest.mark.parametrize("kdm", [4, 5, 9, 15, 16, 22, 31, 32, 41, 47, 48, 63])
+@pytest.mark.parametrize("draw", range(3))
+def test_tiled_remap_on_host_matches_oracle(kdm, draw):
+    """The tiled PPM remap (hyb_tile.h) with 2, 3 and 4 layers per thread,
+    on normal and extreme inputs, against the oracle."""
+    rng = np.random.default_rng(7100 + 31 * draw + kdm)
+    over = _draw(rng, [kdm])
+    over["hybmap"] = 2
+    P, w = abi.workload_params("cfg1", **over)
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=draw, extreme=draw % 2))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, threads=4)
+    hc.run("streaming-tiled", b)
+    df = ol.diff_domains(a, b)
+    assert not df, f"{over}: {df}"
+
+
 def test_streaming_fallback_columns_are_exercised():
     """The in-place check of the streaming remap sends some columns of an
     extreme domain with a maximum layer thickness (dx0k) to the copy-first
