@@ -180,6 +180,67 @@ def mixed_leg(lib, dd, S, sea, args, stream):
             "error_window": "600x400 centre window of the same field"}
 
 
+def bytes_per_column_stokes(K, NZ):
+    """Algorithmic HBM bytes of the Stokes drift remap per sea column: read
+    both snapshots of usdz and vsdz (4*nsdzi), dp at the time level (K) and
+    the bottom pressure; write usdp and vsdp (2K); read ip (int32)."""
+    return 8 * (4 * NZ + K + 1 + 2 * K) + 4
+
+
+def stokes_leg(lib, dd, P, sea, args, stream):
+    """hycom_stokes_vertical_device on the same field's dp, p and ip:
+    nsdzi fixed Stokes layers onto the kdm model layers, seeded snapshots
+    decaying with depth, HIP events on the call's stream."""
+    import numpy as np
+    import torch
+    nz = args.stokes_nsdzi
+    NI, NJ = P.idm + 2 * P.nbdy, P.jdm + 2 * P.nbdy
+    nij = NI * NJ
+    z = np.concatenate([[0.0], np.cumsum(np.geomspace(1.0, 60.0, nz - 1))])
+    sdzi = np.ascontiguousarray(z * abi.ONEM)
+    sdzi[0] = abi.DPTHIN
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+    zmid = torch.tensor(np.concatenate([0.5 * (z[:-1] + z[1:]), [z[-1]]])[:nz], device="cuda")
+    scale = 3.0 + 12.0 * torch.rand(nij, 1, 1, device="cuda", generator=g)
+    prof = torch.exp(-zmid.view(1, 1, nz) / scale)                     # (nij, 1, nz)
+    def snap():
+        amp = 0.1 * torch.randn(nij, 2, 1, device="cuda", generator=g, dtype=torch.float64)
+        x = amp * prof + 0.002 * torch.randn(nij, 2, nz, device="cuda", generator=g,
+                                              dtype=torch.float64)
+        return x.permute(2, 1, 0).contiguous().view(-1)                # (nz, 2, nij) = F order
+    usdz, vsdz = snap(), snap()
+    del prof, scale
+    usdp = torch.zeros(nij * P.kdm, dtype=torch.float64, device="cuda")
+    vsdp = torch.zeros_like(usdp)
+    SP = abi.HybStokesParams(idm=P.idm, jdm=P.jdm, ii=P.ii, jj=P.jj, nbdy=P.nbdy, kdm=P.kdm,
+                             nsdzi=nz, reserved=0, ws0=0.3, ws1=1.0 - 0.3)
+    F = abi.HybStokesFields()
+    F.usdz, F.vsdz, F.sdzi = usdz.data_ptr(), vsdz.data_ptr(), sdzi.ctypes.data
+    F.dp, F.p, F.ip = dd.t["dp"].data_ptr(), dd.t["p"].data_ptr(), dd.t["ip"].data_ptr()
+    F.usdp, F.vsdp = usdp.data_ptr(), vsdp.data_ptr()
+    call = lambda: abi.check(lib.hycom_stokes_vertical_device(ctypes.byref(SP), ctypes.byref(F),
+                                                              2, stream))
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    bpc = bytes_per_column_stokes(P.kdm, nz)
+    out = {"value": sea * P.kdm / (ms * 1e-3), "unit": "column-layers/s", "ms_per_call": ms,
+           "nsdzi": nz, "kdm": P.kdm, "bytes_per_sea_column": bpc,
+           "achieved_GBs": sea * bpc / (ms * 1e-3) / 1e9,
+           "kernels": "k_stokes_blend + k_stokes_walk (walk_core PLM, 2 fields)"}
+    print(f"[bench] stokes {ms:.2f} ms/call", file=sys.stderr, flush=True)
+    del usdz, vsdz, usdp, vsdp
+    torch.cuda.empty_cache()
+    return out
+
+
 def batch_leg(args):
     """64 regional domains of random shape (idm 100-1200, jdm 100-900, kdm
     22/32/41, shelf fraction 0-60%), FP64 PPM, stepped together by
@@ -343,6 +404,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (HYB_PREC_MIXED) leg")
     ap.add_argument("--no-weno", action="store_true", help="skip the WENO-remapper leg")
+    ap.add_argument("--no-stokes", action="store_true",
+                    help="skip the Stokes drift remap leg (hycom_stokes_vertical_device)")
+    ap.add_argument("--stokes-nsdzi", type=int, default=24)
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched ragged domains leg")
     ap.add_argument("--batch-domains", type=int, default=64)
@@ -457,6 +521,9 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
+    stokes = None
+    if not args.no_stokes:
+        stokes = stokes_leg(lib, dd, P, sea, args, stream)
     batch = serve = None
     if not (args.no_batch and args.no_serve):
         del dd  # the GLBb fields make room for the other legs
@@ -520,6 +587,7 @@ def main():
         "cpu_baseline": cpu,
         "weno": weno,
         "mixed": mixed,
+        "stokes": stokes,
         "batch": batch,
         "serve": serve,
     }

@@ -1056,13 +1056,18 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
 // remap of up to NF fields of one column in one walk over the old layers
 // (hybgen_{pcm,plm,ppm,weno}_{coefs,remap}).  fp[f] points at level 1 of
 // field f (stride S); levels > kp take level kp's value (the massless
-// fill); zx = pi(kk+1).  PD levels of every field (and of the old
+// fill); zx = pi(K+1).  PD levels of every field (and of the old
 // interfaces) are loaded ahead of the window, to cover memory latency.
+// K old layers, KO new ones (KO = 0: K).  CDPI: the PLM slopes are taken
+// on dpi = max(dprs, thin) instead of dprs (mod_stokes.F90:683-684 passes
+// dpi to hybgen_plm_coefs; hybgen.F90:1535 passes dprs).
 // ---------------------------------------------------------------------
-template <int METHOD, int NF, int PD, bool FAST = false, class PI, class PO, class SINK>
+template <int METHOD, int NF, int PD, bool FAST = false, bool CDPI = false, class PI, class PO,
+          class SINK>
 HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
-                      int nf, int kp, uint64_t lcm, int64_t S) {
+                      int nf, int kp, uint64_t lcm, int64_t S, int KO = 0) {
   constexpr int PQ = PD > 0 ? PD : 1;
+  const int KN = KO > 0 ? KO : K;
   using CR = typename std::conditional<FAST, float, double>::type;
   const double thin = kDpthin;
   const double dsmll = 1.0e-8;
@@ -1171,6 +1176,25 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
   auto slack = [&](int f) -> double {
     return FAST ? 0x1p-20 * dmax(dabs(smin[f]), dabs(smax[f])) : 0.0;
   };
+  // fold the levels below the window of layer l that are not folded yet:
+  // those loaded ahead, then the rest (idempotent)
+  auto fold_rest = [&](int l) {
+#pragma unroll
+    for (int q = 0; q < PD; q++) {  // levels loaded ahead, not folded yet
+      const int x = l + 3 + q;
+      if (x <= K && x < xl) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+          if (f < nf) (void)fold(f, x, sq[f][q]);
+      }
+    }
+    for (int x = xl; x <= K; x++) {
+#pragma unroll
+      for (int f = 0; f < NF; f++)
+        if (f < nf) (void)load(f, x);
+    }
+    xl = K + 1;
+  };
   // output kc complete: v[f] before the clamp
   auto emit = [&](const double* v) {
     bool certain = true;
@@ -1194,9 +1218,9 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     // ------------------------------------------ coefficients of layer l
     double c1l[NF], c2l[NF], c3l[NF];
     if constexpr (!FAST) {
-      coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, dprs_m, dprs_0, dprs_p,
-                                     dpi_m, dpi_0, dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l,
-                                     c2l, c3l);
+      coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, CDPI ? dpi_m : dprs_m,
+                                     CDPI ? dpi_0 : dprs_0, CDPI ? dpi_p : dprs_p, dpi_m, dpi_0,
+                                     dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l, c2l, c3l);
     } else {
       // mixed mode: the reconstruction in float; the polynomial handed to
       // the FP64 piece integrals keeps the layer mean s(l) (PPM: c3 from
@@ -1244,23 +1268,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     // Every output still to come ends in this layer when it reaches the
     // bottom, and the walk stops here: fold the levels not read yet into
     // the column range now, before any of them is overwritten.
-    if (l < K && pil1 >= zx) {
-#pragma unroll
-      for (int q = 0; q < PD; q++) {  // levels loaded ahead, not folded yet
-        const int x = l + 3 + q;
-        if (x <= K && x < xl) {
-#pragma unroll
-          for (int f = 0; f < NF; f++)
-            if (f < nf) (void)fold(f, x, sq[f][q]);
-        }
-      }
-      for (int x = xl; x <= K; x++) {
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-          if (f < nf) (void)load(f, x);
-      }
-      xl = K + 1;
-    }
+    if (l < K && pil1 >= zx) fold_rest(l);
     // ------------------------------------------ output in progress
     if (accon) {
       if (lb_act == l) {
@@ -1318,7 +1326,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     }
     // ------------------------------------------ outputs starting in layer l
     if (!accon && lt_c == l) {
-      while (kc <= K) {
+      while (kc <= KN) {
         const double zt = zt_c;
         const double zb = dmin(po(kc + 1), zx);
         const bool ends_here = (l == K) || (pil1 >= zb);
@@ -1437,7 +1445,12 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
         }
       }
     }
-    if (kc > K) break;
+    if (kc > KN) {
+      // every output is out; with fewer new layers than old ones the
+      // levels below may not be read yet, and the range covers them all
+      if (KO > 0) fold_rest(l);
+      break;
+    }
     // ------------------------------------------ slide to layer l+1
     const double top = pv4;
     pv0 = pv1;

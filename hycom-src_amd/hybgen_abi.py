@@ -87,6 +87,18 @@ class HybServerStats(ctypes.Structure):
                 ("max_active_seen", ctypes.c_int32), ("active_sum", ctypes.c_double)]
 
 
+class HybStokesParams(ctypes.Structure):
+    _fields_ = [("idm", ctypes.c_int32), ("jdm", ctypes.c_int32), ("ii", ctypes.c_int32),
+                ("jj", ctypes.c_int32), ("nbdy", ctypes.c_int32), ("kdm", ctypes.c_int32),
+                ("nsdzi", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ws0", ctypes.c_double), ("ws1", ctypes.c_double)]
+
+
+class HybStokesFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("usdz", "vsdz", "sdzi", "dp", "p", "ip", "usdp", "vsdp")]
+
+
 # every symbol include/hycom_hybgen.h declares, with its ctypes signature
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -129,6 +141,10 @@ SIGNATURES = {
                                                ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_synth_device": (ctypes.c_int, [_P(HybParams), _P(HybSynth), _P(HybFields),
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "hycom_stokes_vertical": (ctypes.c_int, [_P(HybStokesParams), _P(HybStokesFields),
+                                             ctypes.c_int]),
+    "hycom_stokes_vertical_device": (ctypes.c_int, [_P(HybStokesParams), _P(HybStokesFields),
+                                                    ctypes.c_int, ctypes.c_void_p]),
 }
 
 _lib = None
@@ -430,3 +446,63 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ---------------------------------------------------------------------------
+# Stokes drift onto the model layers (hycom_stokes_vertical*)
+# ---------------------------------------------------------------------------
+ONEM = 9.806 / (1.0 / 1000.0)        # mod_cb_arrays.F90:843-846
+DPTHIN = 0.001 * (ONEM * 0.001)      # 0.001*onemm (mod_stokes.F90:670)
+
+
+class StokesDomain:
+    """Inputs and outputs of the Stokes drift remap (mod_stokes.F90:650-705)
+    over a hybgen domain's dp (time level n), p and ip, as Fortran-ordered
+    numpy arrays.  The two snapshots are seeded profiles that decay with
+    depth over fixed interfaces 0, 1, 2, 3.5, ... m; the first interface is
+    moved to dpthin, as stokes_forfun does (mod_stokes.F90:372-376)."""
+
+    def __init__(self, dom, nsdzi, seed=0, ws0=0.3, n=2):
+        P = dom.P
+        self.dom, self.n = dom, n
+        self.P = HybStokesParams(idm=P.idm, jdm=P.jdm, ii=P.ii, jj=P.jj, nbdy=P.nbdy,
+                                 kdm=P.kdm, nsdzi=nsdzi, reserved=0, ws0=ws0, ws1=1.0 - ws0)
+        NI, NJ = P.idm + 2 * P.nbdy, P.jdm + 2 * P.nbdy
+        rng = np.random.default_rng(seed)
+        z = np.concatenate([[0.0], np.cumsum(np.geomspace(1.0, 60.0, nsdzi - 1))])
+        self.sdzi = np.ascontiguousarray(z * ONEM)
+        self.sdzi[0] = DPTHIN
+        zmid = 0.5 * (z[:-1] + z[1:])
+        zmid = np.concatenate([zmid, [z[-1]]])[:nsdzi]
+        amp = 0.1 * rng.standard_normal((NI, NJ, 2))
+        scale = rng.uniform(3.0, 15.0, (NI, NJ, 1))
+        prof = np.exp(-zmid[None, None, :] / scale)
+        self.usdz = np.asfortranarray(amp[:, :, :, None] * prof[:, :, None, :]
+                                      + 0.002 * rng.standard_normal((NI, NJ, 2, nsdzi)))
+        amp = 0.1 * rng.standard_normal((NI, NJ, 2))
+        self.vsdz = np.asfortranarray(amp[:, :, :, None] * prof[:, :, None, :]
+                                      + 0.002 * rng.standard_normal((NI, NJ, 2, nsdzi)))
+        self.usdp = np.asfortranarray(np.full((NI, NJ, P.kdm), -7.0))
+        self.vsdp = np.asfortranarray(np.full((NI, NJ, P.kdm), 7.0))
+
+    def copy(self):
+        c = StokesDomain.__new__(StokesDomain)
+        c.dom, c.n, c.P = self.dom, self.n, self.P
+        for k in ("sdzi", "usdz", "vsdz", "usdp", "vsdp"):
+            setattr(c, k, getattr(self, k).copy(order="K"))
+        return c
+
+    def fields(self):
+        a = self.dom.a
+        F = HybStokesFields()
+        F.usdz, F.vsdz = self.usdz.ctypes.data, self.vsdz.ctypes.data
+        F.sdzi = self.sdzi.ctypes.data
+        F.dp, F.p, F.ip = a["dp"].ctypes.data, a["p"].ctypes.data, a["ip"].ctypes.data
+        F.usdp, F.vsdp = self.usdp.ctypes.data, self.vsdp.ctypes.data
+        return F
+
+    def run(self):
+        """Copy-mode call through the C ABI (needs a GPU)."""
+        F = self.fields()
+        check(load().hycom_stokes_vertical(ctypes.byref(self.P), ctypes.byref(F), self.n))
+        return self

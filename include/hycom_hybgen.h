@@ -343,6 +343,46 @@ int hycom_hybgen_synth_device(const hyb_params* P, const hyb_synth* S,
                               const hyb_fields* F, int m, int n,
                               void* stream);
 
+/* ---------------------------------------------------------------------
+ * Stokes drift onto the model layers: stokes_vertical_j(n,j)
+ * (mod_stokes.F90:650-705) for j = 1..jj, as stokes_forfun calls it
+ * (mod_stokes.F90:541-546).  At every sea p-point the two snapshots of the
+ * Stokes drift on nsdzi fixed layers are blended,
+ *   s(k) = ws0*usdz(i,j,1,k) + ws1*usdz(i,j,2,k)   (and vsdz),
+ * and remapped by PLM (hybgen_plm_coefs on max(dprs,dpthin), then
+ * hybgen_plm_remap) from the interfaces min(sdzi, p(kk+1)) onto the model
+ * interfaces prsf(k+1) = min(prsf(k) + dp(i,j,k,n), p(kk+1)).  Results go
+ * to usdp, vsdp at sea points; land points are not written.
+ * ------------------------------------------------------------------- */
+typedef struct hyb_stokes_params {
+  int32_t idm, jdm;      /* array extents without halo                      */
+  int32_t ii, jj;        /* active extents                                  */
+  int32_t nbdy;          /* halo width                                      */
+  int32_t kdm;           /* model layers (1..HYB_KMAX)                      */
+  int32_t nsdzi;         /* fixed Stokes interfaces (2..HYB_KMAX)           */
+  int32_t reserved;      /* 0                                               */
+  double  ws0, ws1;      /* snapshot weights (mod_stokes.F90:513-514)       */
+} hyb_stokes_params;
+
+typedef struct hyb_stokes_fields {
+  const double *usdz, *vsdz;  /* (NI,NJ,2,nsdzi) the two snapshots          */
+  const double *sdzi;         /* (nsdzi) interface depths, Pa; host memory  */
+  const double *dp;           /* (NI,NJ,kdm,2)                              */
+  const double *p;            /* (NI,NJ,kdm+1)                              */
+  const int32_t *ip;          /* (NI,NJ)                                    */
+  double *usdp, *vsdp;        /* (NI,NJ,kdm) results                        */
+} hyb_stokes_fields;
+
+/* Drop-in for the loop  do j=1,jj; call stokes_vertical_j(n,j)
+ * (mod_stokes.F90:541-546): host arrays in and out, n the 1-based time
+ * level of dp.  Returns when usdp and vsdp hold the result. */
+int hycom_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n);
+
+/* Same on device pointers (sdzi stays host memory), ordered on `stream`;
+ * returns after enqueueing. */
+int hycom_stokes_vertical_device(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n,
+                                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

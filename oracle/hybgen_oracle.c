@@ -531,6 +531,66 @@ void oracle_remap_column(int method, int kk, const double* si,
   for (int k = 1; k <= kk; k++) so[k - 1] = f[1][k];
 }
 
+/* One column of stokes_vertical_j's remap (mod_stokes.F90:686-694): ks
+ * fields of ki old layers onto ko new ones, PLM slopes on dpi, lcm off.
+ * si[f*ki + k], pi[ki+1], po[ko+1], so[f*ko + k], 0-based. */
+void oracle_plm_dpi_column(int ki, int ko, int ks, const double* si, const double* pi,
+                           const double* po, double thin, double* so) {
+  static __thread colv s[3], f[3], c[3];
+  double pres[KD], prsf[KD], dprs[KD], dpi[KD];
+  int lcm[KD] = {0};
+  for (int i = 1; i <= ks; i++)
+    for (int k = 1; k <= ki; k++) s[i][k] = si[(i - 1) * ki + k - 1];
+  for (int k = 1; k <= ki + 1; k++) pres[k] = pi[k - 1];
+  for (int k = 1; k <= ko + 1; k++) prsf[k] = po[k - 1];
+  for (int k = 1; k <= ki; k++) {
+    dprs[k] = pres[k + 1] - pres[k];
+    dpi[k] = fmax2(dprs[k], thin);
+  }
+  plm_coefs(s, dpi, lcm, c, ki, ks, thin);
+  plm_remap(s, pres, dprs, c, f, prsf, ki, ko, ks, thin);
+  for (int i = 1; i <= ks; i++)
+    for (int k = 1; k <= ko; k++) so[(i - 1) * ko + k - 1] = f[i][k];
+}
+
+/* stokes_vertical_j(n,j) for j = 1..jj (mod_stokes.F90:541-546, 650-705) */
+int oracle_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n) {
+  const int nz = P->nsdzi, kk = P->kdm, nb = P->nbdy;
+  if (nz < 2 || nz > HYB_KMAX || kk < 1 || kk > HYB_KMAX || n < 1 || n > 2) return HYB_EINVAL;
+  const long NI = P->idm + 2 * nb, NIJ = NI * (long)(P->jdm + 2 * nb);
+  const double dpthin = 0.001 * ONEMM;
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= P->jj; j++) {
+    colv s[3], f[3], c[3];
+    double pres[KD], dprs[KD], dpi[KD], prsf[KD];
+    int lcm[KD] = {0};
+    for (int i = 1; i <= P->ii; i++) {
+      const long ij = (long)(i - 1 + nb) + NI * (long)(j - 1 + nb);
+      if (F->ip[ij] == 0) continue;
+      const double pbot = F->p[ij + NIJ * kk];
+      pres[1] = 0.0;
+      for (int k = 1; k <= nz; k++) {
+        const long a = ij + NIJ * (long)(2 * (k - 1));
+        s[1][k] = P->ws0 * F->usdz[a] + P->ws1 * F->usdz[a + NIJ];
+        s[2][k] = P->ws0 * F->vsdz[a] + P->ws1 * F->vsdz[a + NIJ];
+        pres[k + 1] = fmin2(F->sdzi[k - 1], pbot);
+        dprs[k] = pres[k + 1] - pres[k];
+        dpi[k] = fmax2(dprs[k], dpthin);
+      }
+      prsf[1] = 0.0;
+      for (int k = 1; k <= kk; k++)
+        prsf[k + 1] = fmin2(prsf[k] + F->dp[ij + NIJ * ((long)(k - 1) + (long)kk * (n - 1))], pbot);
+      plm_coefs(s, dpi, lcm, c, nz, 2, dpthin);
+      plm_remap(s, pres, dprs, c, f, prsf, nz, kk, 2, dpthin);
+      for (int k = 1; k <= kk; k++) {
+        F->usdp[ij + NIJ * (long)(k - 1)] = f[1][k];
+        F->vsdp[ij + NIJ * (long)(k - 1)] = f[2][k];
+      }
+    }
+  }
+  return 0;
+}
+
 /* ---------------------------------------------------------------------- */
 /* field access                                                            */
 /* ---------------------------------------------------------------------- */

@@ -35,6 +35,16 @@ int oracle_hybgen_omp(const hyb_params* P, const hyb_fields* F,
 /* sigma-2, 17-term EOS (stmt_fns.h:505-511). */
 double oracle_sig(double t, double s);
 
+/* stokes_vertical_j(n,j) for j = 1..jj (mod_stokes.F90:541-546, 650-705)
+ * on host arrays: usdp, vsdp at sea points. */
+int oracle_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n);
+
+/* One column of that remap: ks fields (<= 2) of ki old layers onto ko new
+ * ones, PLM slopes on dpi = max(dprs, thin), no PCM layers.
+ * si[f*ki + k], pi[ki+1], po[ko+1], so[f*ko + k]. */
+void oracle_plm_dpi_column(int ki, int ko, int ks, const double* si, const double* pi,
+                           const double* po, double thin, double* so);
+
 /* Single-column remappers on caller arrays, 1 field (ks=1), for unit
  * tests: method 0 PCM, 1 PLM, 2 PPM, 3 WENO.  si[k], pi[k+1], po[k+1],
  * lcm[k] (0/1), so[k] are 0-based C arrays of the Fortran 1-based ones. */

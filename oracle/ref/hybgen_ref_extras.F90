@@ -87,4 +87,33 @@
       enddo
       end subroutine href_remap
 
+! One column of the remap stokes_vertical_j makes (mod_stokes.F90:686-694):
+! ks fields of ki fixed layers onto ko model layers, hybgen_plm_coefs on
+! dpi = max(dprs,thin) with no PCM layers, then hybgen_plm_remap.  The
+! blend and the interfaces are the caller's (si, pin, po).
+      subroutine href_plm_dpi_column(ki, ko, ks, si, pin, po, thin, so) &
+          bind(C, name='href_plm_dpi_column')
+      integer(c_int), value :: ki, ko, ks
+      real(c_double), intent(in) :: si(ki,ks), pin(ki+1), po(ko+1)
+      real(c_double), value :: thin
+      real(c_double), intent(out) :: so(ko,ks)
+      real :: s1(ki,ks), f1(ko,ks), c1d(ki,ks), dprs(ki), dpi(ki)
+      real :: pi1(ki+1), po1(ko+1), th
+      logical :: lcm(ki)
+      integer :: k
+      external hybgen_plm_coefs, hybgen_plm_remap
+      th = thin
+      s1 = si
+      pi1 = pin
+      po1 = po
+      lcm(:) = .false.
+      do k= 1,ki
+        dprs(k) = pi1(k+1)-pi1(k)
+        dpi(k)  = max(dprs(k),th)
+      enddo
+      call hybgen_plm_coefs(s1,dpi,lcm,c1d,ki,ks,th)
+      call hybgen_plm_remap(s1,pi1,dprs,c1d,f1,po1,ki,ko,ks,th)
+      so = f1
+      end subroutine href_plm_dpi_column
+
       end module hybgen_ref_extras

@@ -432,3 +432,39 @@ extern "C" void hostcheck_tile_column(int kk, const double* si, const double* pi
   else tile_row<4>(P, V, W, 1, 1);
   for (int k = 0; k < kk; k++) so[k] = temp[k * NIJ];
 }
+
+// ---- Stokes drift onto the model layers (hyb_stokes.h) ----
+#include "hyb_stokes.h"
+
+extern "C" int hostcheck_stokes(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n) {
+  KStokes Q{};
+  Q.ii = P->ii;
+  Q.jj = P->jj;
+  Q.kk = P->kdm;
+  Q.nsdzi = P->nsdzi;
+  Q.nb = P->nbdy;
+  Q.n = n;
+  Q.NI = (int64_t)P->idm + 2 * P->nbdy;
+  Q.NIJ = Q.NI * ((int64_t)P->jdm + 2 * P->nbdy);
+  Q.ws0 = P->ws0;
+  Q.ws1 = P->ws1;
+  for (int k = 0; k < P->nsdzi; k++) Q.sdzi[k] = F->sdzi[k];
+  Q.usdz = F->usdz;
+  Q.vsdz = F->vsdz;
+  Q.dp = F->dp;
+  Q.p = F->p;
+  Q.ip = F->ip;
+  Q.usdp = F->usdp;
+  Q.vsdp = F->vsdp;
+  std::vector<double> blend((size_t)Q.NIJ * 2 * Q.nsdzi, 0.0);
+  Q.blend = blend.data();
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 1; j <= Q.jj; j++)
+    for (int i = 1; i <= Q.ii; i++) {
+      const int64_t c0 = stokes_ij(Q, i, j);
+      if (!Q.ip[c0]) continue;
+      for (int k = 1; k <= Q.nsdzi; k++) stokes_blend(Q, c0, k);
+      stokes_column<2>(Q, c0);
+    }
+  return 0;
+}

@@ -26,6 +26,8 @@ def lib():
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
         h.hostcheck_stream_unsafe_columns.restype = ctypes.c_int64
         h.hostcheck_tile_column.restype = None
+        h.hostcheck_stokes.argtypes = [ctypes.POINTER(abi.HybStokesParams),
+                                       ctypes.POINTER(abi.HybStokesFields), ctypes.c_int]
         h.hostcheck_stream_uv_fallback_points.restype = ctypes.c_int64
         _lib = h
     return _lib

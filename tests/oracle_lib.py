@@ -31,6 +31,12 @@ def oracle():
         lib.oracle_remap_column.argtypes = [
             ctypes.c_int, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
             _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_double, _P(ctypes.c_double)]
+        lib.oracle_stokes_vertical.argtypes = [_P(abi.HybStokesParams),
+                                               _P(abi.HybStokesFields), ctypes.c_int]
+        lib.oracle_plm_dpi_column.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
+            _P(ctypes.c_double), _P(ctypes.c_double), ctypes.c_double, _P(ctypes.c_double)]
+        lib.oracle_plm_dpi_column.restype = None
         _oracle = lib
     return _oracle
 
@@ -55,6 +61,10 @@ def ref(denfix=False):
         lib.href_remap.argtypes = [
             ctypes.c_int, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
             _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_double, _P(ctypes.c_double)]
+        lib.href_plm_dpi_column.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
+            _P(ctypes.c_double), _P(ctypes.c_double), ctypes.c_double, _P(ctypes.c_double)]
+        lib.href_plm_dpi_column.restype = None
         _refs[denfix] = lib
     return _refs[denfix]
 
@@ -92,3 +102,11 @@ def diff_domains(a, b, names=abi.OUTPUTS):
         if c:
             out[n] = c
     return out
+
+
+def run_oracle_stokes(sd):
+    F = sd.fields()
+    rc = oracle().oracle_stokes_vertical(ctypes.byref(sd.P), ctypes.byref(F), sd.n)
+    if rc != 0:
+        raise RuntimeError(f"oracle_stokes_vertical rc={rc}")
+    return sd
