@@ -241,6 +241,56 @@ def stokes_leg(lib, dd, P, sea, args, stream):
     return out
 
 
+def bytes_per_column_trc(KI, KO, KS):
+    """Algorithmic HBM bytes of the tracer column remap per masked column:
+    read si (ki*ks), pi (ki+1) and po (ko+1), write so (ko*ks), read the
+    mask (int32)."""
+    return 8 * (KI * KS + KI + 1 + KO + 1 + KO * KS) + 4
+
+
+def trc_leg(lib, dd, P, sea, args, stream):
+    """hycom_trc_remap_device with plmtrc (method 1), as thermf's tracer
+    relaxation calls it (thermf.F90:770-806): the field's tracers as seeded
+    climatology on interfaces half a layer off the model's, onto the
+    model's interfaces p, at every sea point."""
+    import torch
+    NI, NJ = P.idm + 2 * P.nbdy, P.jdm + 2 * P.nbdy
+    nij, K, T = NI * NJ, P.kdm, max(P.ntracr, 1)
+    p = dd.t["p"].view(K + 1, nij)
+    pi = torch.empty_like(p)
+    pi[0] = 0.0
+    pi[1:K] = 0.5 * (p[1:K] + p[2:K + 1])
+    pi[K] = p[K]
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 7)
+    si = torch.cumsum(torch.randn(T, K, nij, device="cuda", generator=g, dtype=torch.float64),
+                      dim=1) + 5.0
+    so = torch.zeros(T, K, nij, device="cuda", dtype=torch.float64)
+    TP = abi.HybTrcParams(idm=P.idm, jdm=P.jdm, ii=P.ii, jj=P.jj, nbdy=P.nbdy, ki=K, ko=K, ks=T,
+                          method=1, reserved=0)
+    F = abi.HybTrcFields()
+    F.si, F.pi, F.po = si.data_ptr(), pi.data_ptr(), dd.t["p"].data_ptr()
+    F.mask, F.so = dd.t["ip"].data_ptr(), so.data_ptr()
+    call = lambda: abi.check(lib.hycom_trc_remap_device(ctypes.byref(TP), ctypes.byref(F), stream))
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    bpc = bytes_per_column_trc(K, K, T)
+    out = {"value": sea * K / (ms * 1e-3), "unit": "column-layers/s", "ms_per_call": ms,
+           "method": "plmtrc", "ki": K, "ko": K, "ks": T, "bytes_per_sea_column": bpc,
+           "achieved_GBs": sea * bpc / (ms * 1e-3) / 1e9, "kernels": "k_trc"}
+    print(f"[bench] trc {ms:.2f} ms/call", file=sys.stderr, flush=True)
+    del si, so, pi
+    torch.cuda.empty_cache()
+    return out
+
+
 def batch_leg(args):
     """64 regional domains of random shape (idm 100-1200, jdm 100-900, kdm
     22/32/41, shelf fraction 0-60%), FP64 PPM, stepped together by
@@ -407,6 +457,8 @@ def main():
     ap.add_argument("--no-stokes", action="store_true",
                     help="skip the Stokes drift remap leg (hycom_stokes_vertical_device)")
     ap.add_argument("--stokes-nsdzi", type=int, default=24)
+    ap.add_argument("--no-trc", action="store_true",
+                    help="skip the tracer column remap leg (hycom_trc_remap_device, plmtrc)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched ragged domains leg")
     ap.add_argument("--batch-domains", type=int, default=64)
@@ -521,9 +573,11 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
-    stokes = None
+    stokes = trc = None
     if not args.no_stokes:
         stokes = stokes_leg(lib, dd, P, sea, args, stream)
+    if not args.no_trc:
+        trc = trc_leg(lib, dd, P, sea, args, stream)
     batch = serve = None
     if not (args.no_batch and args.no_serve):
         del dd  # the GLBb fields make room for the other legs
@@ -588,6 +642,7 @@ def main():
         "weno": weno,
         "mixed": mixed,
         "stokes": stokes,
+        "trc": trc,
         "batch": batch,
         "serve": serve,
     }

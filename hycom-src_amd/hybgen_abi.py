@@ -99,6 +99,15 @@ class HybStokesFields(ctypes.Structure):
                 ("usdz", "vsdz", "sdzi", "dp", "p", "ip", "usdp", "vsdp")]
 
 
+class HybTrcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("idm", "jdm", "ii", "jj", "nbdy", "ki", "ko", "ks", "method", "reserved")]
+
+
+class HybTrcFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("si", "pi", "po", "mask", "so")]
+
+
 # every symbol include/hycom_hybgen.h declares, with its ctypes signature
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -145,6 +154,9 @@ SIGNATURES = {
                                              ctypes.c_int]),
     "hycom_stokes_vertical_device": (ctypes.c_int, [_P(HybStokesParams), _P(HybStokesFields),
                                                     ctypes.c_int, ctypes.c_void_p]),
+    "hycom_trc_remap": (ctypes.c_int, [_P(HybTrcParams), _P(HybTrcFields)]),
+    "hycom_trc_remap_device": (ctypes.c_int, [_P(HybTrcParams), _P(HybTrcFields),
+                                              ctypes.c_void_p]),
 }
 
 _lib = None

@@ -379,9 +379,48 @@ typedef struct hyb_stokes_fields {
 int hycom_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n);
 
 /* Same on device pointers (sdzi stays host memory), ordered on `stream`;
- * returns after enqueueing. */
+ * returns after enqueueing.  Calls on one device share a workspace: do not
+ * overlap them on different streams. */
 int hycom_stokes_vertical_device(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n,
                                  void* stream);
+
+/* ---------------------------------------------------------------------
+ * Tracer column remaps of trcupd.F90: pcmtrc (808-909, method 0) and
+ * plmtrc with its slopes plmtrcx/plmtrcs (911-1147, method 1), applied at
+ * every point where mask != 0, as initrc (trcupd.F90:167-199) and the
+ * tracer relaxation of thermf (thermf.F90:770-806) call them per sea
+ * column: si(ki,ks) on the interfaces pi(ki+1) onto so(ko,ks) on po(ko+1).
+ * Interfaces are non-decreasing, pi(1) = 0 = po(1).  Where the reference
+ * would stop ('bad lf', trcupd.F90:866-869: po out of order) the column's
+ * later outputs are NaN and the call returns HYB_EINVAL (host call) after
+ * writing the rest.  A thin first output (po(2)-po(1) < 1e-6) takes si(1);
+ * the reference reads outside its array there.
+ * ------------------------------------------------------------------- */
+typedef struct hyb_trc_params {
+  int32_t idm, jdm;      /* array extents without halo                      */
+  int32_t ii, jj;        /* active extents                                  */
+  int32_t nbdy;          /* halo width                                      */
+  int32_t ki, ko;        /* input and output layers (2..HYB_KMAX, 1..HYB_KMAX) */
+  int32_t ks;            /* fields (1..HYB_TRMAX)                           */
+  int32_t method;        /* 0 pcmtrc, 1 plmtrc                              */
+  int32_t reserved;      /* 0                                               */
+} hyb_trc_params;
+
+typedef struct hyb_trc_fields {
+  const double *si;      /* (NI,NJ,ki,ks)                                   */
+  const double *pi;      /* (NI,NJ,ki+1)                                    */
+  const double *po;      /* (NI,NJ,ko+1)                                    */
+  const int32_t *mask;   /* (NI,NJ): columns remapped where != 0            */
+  double *so;            /* (NI,NJ,ko,ks) results at masked points          */
+} hyb_trc_fields;
+
+/* Host arrays in and out; returns when so holds the result. */
+int hycom_trc_remap(const hyb_trc_params* P, const hyb_trc_fields* F);
+
+/* Same on device pointers, ordered on `stream`; returns after enqueueing
+ * (a column that would stop is left NaN, not reported).  Calls on one
+ * device share a workspace: do not overlap them on different streams. */
+int hycom_trc_remap_device(const hyb_trc_params* P, const hyb_trc_fields* F, void* stream);
 
 #ifdef __cplusplus
 }

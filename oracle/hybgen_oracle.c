@@ -553,6 +553,140 @@ void oracle_plm_dpi_column(int ki, int ko, int ks, const double* si, const doubl
     for (int k = 1; k <= ko; k++) so[(i - 1) * ko + k - 1] = f[i][k];
 }
 
+/* ---------------------------------------------------------------------- */
+/* trcupd.F90 tracer column remaps: pcmtrc, plmtrc, plmtrcx, plmtrcs       */
+/* ---------------------------------------------------------------------- */
+#define TRC_THIN 1.e-6 /* trcupd.F90:857, 960 */
+
+/* plmtrcs, trcupd.F90:1088-1147: slopes of one field (1-based a, s) */
+static void trc_plmtrcs(const double* rl, const double* rr, const double* a, double* s, int n) {
+  s[1] = 0.0;
+  for (int j = 2; j <= n - 1; j++) {
+    double sl = rl[j] * (a[j] - a[j - 1]);
+    double sr = rr[j] * (a[j + 1] - a[j]);
+    if (sl * sr > 0.0) s[j] = copysign(fmin2(fabs(sl), fabs(sr)), sl);
+    else s[j] = 0.0;
+  }
+  s[n] = 0.0;
+  for (int j = 2; j <= n - 1; j++) {
+    if (s[j] != 0.0) {
+      double dl = -0.5 * (s[j] + s[j - 1]) + a[j] - a[j - 1];
+      double dr = -0.5 * (s[j + 1] + s[j]) + a[j + 1] - a[j];
+      double ds = copysign(fmin2(fabs(dl), fabs(dr)), dl);
+      s[j] = s[j] + 2.0 * ds;
+    }
+  }
+}
+
+/* pcmtrc (method 0, trcupd.F90:808-909) or plmtrc (method 1, 911-1036)
+ * of one column: si[f*ki + l-1], pi[ki+1], po[ko+1], so[f*ko + k-1].
+ * Returns -1 where the reference stops ('bad lf'); the outputs from there
+ * on are NaN.  A thin first output takes si(1): the reference reads so(0)
+ * there, outside the array. */
+int oracle_trc_column(int method, int ki, int ks, const double* si, const double* pi,
+                      const double* po, int ko, double* so) {
+  double p[KD], pit[KD], ql[KD], qr[KD], a[KD], sis[KD], sok[HYB_TRMAX + 5];
+  double smin[HYB_TRMAX + 5], smax[HYB_TRMAX + 5];
+  static __thread double slopes[HYB_TRMAX + 5][KD];
+  int rc = 0;
+  for (int l = 1; l <= ki + 1; l++) p[l] = pi[l - 1];
+  if (method == 1) {
+    for (int i = 1; i <= ks; i++) {
+      smin[i] = si[(i - 1) * ki];
+      smax[i] = si[(i - 1) * ki];
+      for (int l = 2; l <= ki; l++) {
+        smin[i] = fmin2(smin[i], si[(i - 1) * ki + l - 1]);
+        smax[i] = fmax2(smax[i], si[(i - 1) * ki + l - 1]);
+      }
+    }
+    for (int l = 1; l <= ki; l++) pit[l] = fmax2(p[l + 1] - p[l], TRC_THIN);
+    /* plmtrcx, trcupd.F90:1038-1086 */
+    ql[1] = qr[1] = 0.0;
+    for (int l = 2; l <= ki - 1; l++) {
+      ql[l] = 2.0 * pit[l] / (pit[l - 1] + pit[l]);
+      qr[l] = 2.0 * pit[l] / (pit[l] + pit[l + 1]);
+    }
+    ql[ki] = qr[ki] = 0.0;
+    for (int i = 1; i <= ks; i++) {
+      for (int l = 1; l <= ki; l++) a[l] = si[(i - 1) * ki + l - 1];
+      trc_plmtrcs(ql, qr, a, sis, ki);
+      for (int l = 1; l <= ki; l++) slopes[i][l] = sis[l];
+    }
+  }
+  int lf = 1;
+  double zb = po[0];
+  for (int k = 1; k <= ko; k++) {
+    double zt = zb;
+    zb = po[k];
+    if (zb - zt < TRC_THIN || zt >= p[ki + 1]) {
+      for (int i = 1; i <= ks; i++)
+        so[(i - 1) * ko + k - 1] = (k > 1) ? so[(i - 1) * ko + k - 2] : si[(i - 1) * ki];
+      continue;
+    }
+    if (p[lf] > zt) {
+      for (int i = 1; i <= ks; i++) so[(i - 1) * ko + k - 1] = NAN;
+      rc = -1;
+      continue;
+    }
+    for (int i = 1; i <= ks; i++) sok[i] = 0.0;
+    for (int l = lf; l <= ki; l++) {
+      if (p[l] > zb) {
+        lf = l - 1;
+        break;
+      } else if (p[l] >= zt && p[l + 1] <= zb) {
+        double q = fmax2(p[l + 1] - p[l], TRC_THIN) / (zb - zt);
+        for (int i = 1; i <= ks; i++) sok[i] = sok[i] + q * si[(i - 1) * ki + l - 1];
+      } else {
+        double q = fmax2(fmin2(p[l + 1], zb) - fmax2(p[l], zt), TRC_THIN) / (zb - zt);
+        if (method == 0) {
+          for (int i = 1; i <= ks; i++) sok[i] = sok[i] + q * si[(i - 1) * ki + l - 1];
+        } else {
+          double zc = 0.5 * (fmin2(p[l + 1], zb) + fmax2(p[l], zt));
+          double qc = (zc - p[l]) / pit[l] - 0.5;
+          for (int i = 1; i <= ks; i++)
+            sok[i] = sok[i] + q * (si[(i - 1) * ki + l - 1] + qc * slopes[i][l]);
+        }
+      }
+    }
+    for (int i = 1; i <= ks; i++) {
+      double v = sok[i];
+      if (method == 1) {
+        v = fmax2(smin[i], v);
+        v = fmin2(smax[i], v);
+      }
+      so[(i - 1) * ko + k - 1] = v;
+    }
+  }
+  return rc;
+}
+
+/* The column remap at every masked point of a grid (initrc trcupd.F90:
+ * 167-199, thermf.F90:770-806): si (NI,NJ,ki,ks), pi (NI,NJ,ki+1), po
+ * (NI,NJ,ko+1), so (NI,NJ,ko,ks). */
+int oracle_trc_remap(const hyb_trc_params* P, const hyb_trc_fields* F) {
+  const int ki = P->ki, ko = P->ko, ks = P->ks, nb = P->nbdy;
+  if (ki < 2 || ki > HYB_KMAX || ko < 1 || ko > HYB_KMAX || ks < 1 || ks > HYB_TRMAX)
+    return HYB_EINVAL;
+  const long NI = P->idm + 2 * nb, NIJ = NI * (long)(P->jdm + 2 * nb);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int j = 1; j <= P->jj; j++) {
+    double si[HYB_TRMAX * KD], pi[KD], po[KD], so[HYB_TRMAX * KD];
+    for (int i = 1; i <= P->ii; i++) {
+      const long ij = (long)(i - 1 + nb) + NI * (long)(j - 1 + nb);
+      if (F->mask[ij] == 0) continue;
+      for (int f = 0; f < ks; f++)
+        for (int l = 0; l < ki; l++) si[f * ki + l] = F->si[ij + NIJ * ((long)l + (long)ki * f)];
+      for (int l = 0; l <= ki; l++) pi[l] = F->pi[ij + NIJ * l];
+      for (int k = 0; k <= ko; k++) po[k] = F->po[ij + NIJ * k];
+      if (oracle_trc_column(P->method, ki, ks, si, pi, po, ko, so)) bad = 1;
+      for (int f = 0; f < ks; f++)
+        for (int k = 0; k < ko; k++) F->so[ij + NIJ * ((long)k + (long)ko * f)] = so[f * ko + k];
+    }
+  }
+  return bad ? HYB_EINVAL : 0;
+}
+
 /* stokes_vertical_j(n,j) for j = 1..jj (mod_stokes.F90:541-546, 650-705) */
 int oracle_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n) {
   const int nz = P->nsdzi, kk = P->kdm, nb = P->nbdy;

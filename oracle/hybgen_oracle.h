@@ -35,6 +35,16 @@ int oracle_hybgen_omp(const hyb_params* P, const hyb_fields* F,
 /* sigma-2, 17-term EOS (stmt_fns.h:505-511). */
 double oracle_sig(double t, double s);
 
+/* pcmtrc (method 0, trcupd.F90:808-909) or plmtrc (method 1, 911-1147) of
+ * one column: si[f*ki + l], pi[ki+1], po[ko+1], so[f*ko + k].  Returns -1
+ * where the reference stops ('bad lf', its outputs NaN from there). */
+int oracle_trc_column(int method, int ki, int ks, const double* si, const double* pi,
+                      const double* po, int ko, double* so);
+
+/* The same at every point of F->mask != 0 (initrc trcupd.F90:167-199,
+ * thermf.F90:770-806); HYB_EINVAL if any column would stop. */
+int oracle_trc_remap(const hyb_trc_params* P, const hyb_trc_fields* F);
+
 /* stokes_vertical_j(n,j) for j = 1..jj (mod_stokes.F90:541-546, 650-705)
  * on host arrays: usdp, vsdp at sea points. */
 int oracle_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n);

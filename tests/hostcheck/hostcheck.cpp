@@ -468,3 +468,34 @@ extern "C" int hostcheck_stokes(const hyb_stokes_params* P, const hyb_stokes_fie
     }
   return 0;
 }
+
+// ---- tracer column remaps of trcupd.F90 (hyb_trc.h) ----
+#include "hyb_trc.h"
+
+extern "C" int hostcheck_trc(const hyb_trc_params* P, const hyb_trc_fields* F) {
+  KTrc Q{};
+  Q.ii = P->ii;
+  Q.jj = P->jj;
+  Q.ki = P->ki;
+  Q.ko = P->ko;
+  Q.ks = P->ks;
+  Q.method = P->method;
+  Q.nb = P->nbdy;
+  Q.NI = (int64_t)P->idm + 2 * P->nbdy;
+  Q.NIJ = Q.NI * ((int64_t)P->jdm + 2 * P->nbdy);
+  Q.si = F->si;
+  Q.pi = F->pi;
+  Q.po = F->po;
+  Q.mask = F->mask;
+  Q.so = F->so;
+  std::vector<double> slope((size_t)Q.NIJ * Q.ki * kTrcChunk, 0.0);
+  Q.slope = slope.data();
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+  for (int j = 1; j <= Q.jj; j++)
+    for (int i = 1; i <= Q.ii; i++) {
+      const int64_t c0 = trc_ij(Q, i, j);
+      if (Q.mask[c0] && !trc_column(Q, c0)) bad = 1;
+    }
+  return bad ? HYB_EINVAL : 0;
+}

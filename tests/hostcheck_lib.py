@@ -28,6 +28,8 @@ def lib():
         h.hostcheck_tile_column.restype = None
         h.hostcheck_stokes.argtypes = [ctypes.POINTER(abi.HybStokesParams),
                                        ctypes.POINTER(abi.HybStokesFields), ctypes.c_int]
+        h.hostcheck_trc.argtypes = [ctypes.POINTER(abi.HybTrcParams),
+                                    ctypes.POINTER(abi.HybTrcFields)]
         h.hostcheck_stream_uv_fallback_points.restype = ctypes.c_int64
         _lib = h
     return _lib

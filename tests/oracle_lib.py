@@ -37,6 +37,10 @@ def oracle():
             ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
             _P(ctypes.c_double), _P(ctypes.c_double), ctypes.c_double, _P(ctypes.c_double)]
         lib.oracle_plm_dpi_column.restype = None
+        lib.oracle_trc_column.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
+            _P(ctypes.c_double), ctypes.c_int, _P(ctypes.c_double)]
+        lib.oracle_trc_remap.argtypes = [_P(abi.HybTrcParams), _P(abi.HybTrcFields)]
         _oracle = lib
     return _oracle
 
@@ -110,3 +114,27 @@ def run_oracle_stokes(sd):
     if rc != 0:
         raise RuntimeError(f"oracle_stokes_vertical rc={rc}")
     return sd
+
+
+TRC_REF_SO = os.path.join(REF_DIR, "libtrcupd_ref.so")
+_trc_ref = None
+
+
+def trc_ref_available():
+    return os.path.exists(TRC_REF_SO)
+
+
+def trc_ref():
+    """trcupd.F90 as built by oracle/ref: opened with lazy binding, since
+    routines of it that the tests never call use routines of other files
+    (pcmtrc_ and plmtrc_ need none of them)."""
+    global _trc_ref
+    if _trc_ref is None:
+        lib = ctypes.CDLL(TRC_REF_SO, mode=os.RTLD_LAZY)
+        for fn in (lib.pcmtrc_, lib.plmtrc_):
+            fn.restype = None
+            fn.argtypes = [_P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_int),
+                           _P(ctypes.c_int), _P(ctypes.c_double), _P(ctypes.c_double),
+                           _P(ctypes.c_int)]
+        _trc_ref = lib
+    return _trc_ref
