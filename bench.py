@@ -478,6 +478,9 @@ def main():
                     help="streaming u/v prefetch depth 0 or 2 (knob 1)")
     ap.add_argument("--regrid-ahead", type=int, default=None,
                     help="regrid window loaded one iteration ahead (1) or not (0; knob 3)")
+    ap.add_argument("--concurrent-uv", type=int, default=None,
+                    help="u/v phase beside the p-column remap on a second stream, 0 or 1 "
+                         "(knob 4)")
     ap.add_argument("--uv-recompute", type=int, default=None,
                     help="u/v new thicknesses from LDS (0) or formed again from p (1; knob 2)")
     args = ap.parse_args()
@@ -498,6 +501,8 @@ def main():
         abi.check(lib.hycom_hybgen_tune(2, args.uv_recompute))
     if args.regrid_ahead is not None:
         abi.check(lib.hycom_hybgen_tune(3, args.regrid_ahead))
+    if args.concurrent_uv is not None:
+        abi.check(lib.hycom_hybgen_tune(4, args.concurrent_uv))
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     P, w = abi.workload_params(args.workload, hybmap=args.hybmap)

@@ -747,6 +747,11 @@ int hycom_hybgen_tune(int knob, int value) {
     g_tune.regrid_ahead = value;
     return 0;
   }
+  if (knob == 4) {
+    if (value != 0 && value != 1) return fail(HYB_EINVAL, "concurrent u/v %d: 0 or 1", value);
+    g_tune.concurrent_uv = value;
+    return 0;
+  }
   return fail(HYB_EINVAL, "tuning knob %d", knob);
 }
 

@@ -1,9 +1,40 @@
 // hyb_k_stream.hip -- the streaming family (hyb_stream.h): regrid, u/v
 // kernels and the launcher of the whole call, any kdm <= 64.
+#include <mutex>
+
 #include "hyb_launch.h"
 #include "hyb_stream.h"
 
 namespace hyb {
+
+// A second stream per device for the u/v phase, which needs only the
+// regrid's interfaces and runs beside the p-column remap: the two kernels
+// touch disjoint outputs and are both bound by latency at low occupancy.
+// The fork/join events order it after the regrid and before whatever the
+// caller enqueues next; the mutex keeps one call's record/wait pairs
+// together when several host threads call at once.
+namespace {
+struct Fork {
+  hipStream_t s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_fork_mu;
+Fork g_fork[64];
+hipError_t fork_of(Fork** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  Fork& F = g_fork[dev];
+  if (!F.s2) {
+    if ((e = hipStreamCreateWithFlags(&F.s2, hipStreamNonBlocking)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&F.fork, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&F.join, hipEventDisableTiming)) != hipSuccess) return e;
+  }
+  *out = &F;
+  return hipSuccess;
+}
+}  // namespace
 
 __global__ void __launch_bounds__(kBlock) k_regrid_s(KParams P, View V, Scratch W, int n) {
   int j = blockIdx.y + 1;
@@ -81,6 +112,9 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   dim3 blk(kBlock);
   dim3 g2((P.ii + kAlignPad + kBlock - 1) / kBlock, P.jj);
   launch_diag_pre(P, V, n, s);
+  std::unique_lock<std::mutex> fork_lock(g_fork_mu, std::defer_lock);
+  Fork* F = nullptr;
+  hipStream_t su = s;  // stream of the u/v phase
   {
     PhaseMark pm(T, s, 0);
     {
@@ -89,6 +123,20 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
         hipLaunchKernelGGL(k_regrid_a, g2, blk, 0, s, P, V, W, n);
       else
         hipLaunchKernelGGL(k_regrid_s, g2, blk, 0, s, P, V, W, n);
+    }
+    if (tune.concurrent_uv) {
+      hipError_t e = fork_of(&F);
+      if (e != hipSuccess) return e;
+      fork_lock.lock();
+      if ((e = hipEventRecord(F->fork, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(F->s2, F->fork, 0)) != hipSuccess) return e;
+      su = F->s2;
+      {
+        PhaseMark pu(T, su, 1);
+        if ((e = launch_halo(P, V, su)) != hipSuccess) return e;
+      }
+      PhaseMark pu(T, su, 2);
+      launch_uv_stream(P, V, W, n, tune, su);
     }
     {
       PhaseMark pk(T, s, 7);
@@ -107,12 +155,16 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   }
   launch_diag_post(P, V, n, s);
   hipError_t e;
-  {
-    PhaseMark pm(T, s, 1);
-    e = launch_halo(P, V, s);
-  }
-  if (e != hipSuccess) return e;
-  {
+  if (F) {  // join: the caller's stream waits for the u/v phase
+    if ((e = hipEventRecord(F->join, su)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s, F->join, 0)) != hipSuccess) return e;
+    fork_lock.unlock();
+  } else {
+    {
+      PhaseMark pm(T, s, 1);
+      e = launch_halo(P, V, s);
+    }
+    if (e != hipSuccess) return e;
     PhaseMark pm(T, s, 2);
     launch_uv_stream(P, V, W, n, tune, s);
   }

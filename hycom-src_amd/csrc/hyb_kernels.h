@@ -42,6 +42,7 @@ struct StreamTuning {
   int uv_prefetch = 2;
   int uv_recompute = 1;  // u/v new thicknesses formed again from p, no LDS (fastest measured)
   int regrid_ahead = 1;  // the regrid's window loaded one iteration ahead (fastest measured)
+  int concurrent_uv = 0; // the u/v phase on a second stream beside the p-column remap
 };
 // Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
 // plus uvf (2*NIJ int32).

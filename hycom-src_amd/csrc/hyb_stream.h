@@ -919,14 +919,15 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
       c2l[f] = R(0);
       c3l[f] = R(0);
       if (f < nf) {
+        // the per-lane choices below are selects, not branches: the
+        // values are those of the reference's taken branch
         R as_n = R(0), al_n = R(0);
-        if (do_as) {
+        {
           const R slj = s2[f] - s1[f];
           const R srj = s3[f] - s2[f];
-          if (slj * srj > R(0)) {
-            R scj = dpq3 * (dp2jp_l * srj * qdpjp_n + dpj2p_n * slj * st.qdpjp_l);
-            as_n = gsign(gmin3(gabs(R(2) * slj), gabs(scj), gabs(R(2) * srj)), scj);
-          }
+          const R scj = dpq3 * (dp2jp_l * srj * qdpjp_n + dpj2p_n * slj * st.qdpjp_l);
+          const R asv = gsign(gmin3(gabs(R(2) * slj), gabs(scj), gabs(R(2) * srj)), scj);
+          as_n = (do_as && slj * srj > R(0)) ? asv : R(0);
         }
         if (jn == K) {
           al_n = s2[f];  // al(kk) = s(kk)
@@ -944,29 +945,22 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
         else if (l == 1) ar = s1[f];  // ar(1) = s(1)
         else ar = al_n;               // ar(l) = al(l+1)
         if (lim) {
-          if (lim_flat) {
-            al = s1[f];
-            ar = s1[f];
-          } else if ((s2[f] - s1[f]) * (s1[f] - s0[f]) <= R(0)) {
-            al = s1[f];
-            ar = s1[f];
-          } else {
-            const R da = ar - al;
-            const R a6 = R(6) * s1[f] - R(3) * (al + ar);
-            if (da * a6 > da * da) {
-              al = R(3) * s1[f] - R(2) * ar;
-            } else if (da * a6 < -(da * da)) {
-              ar = R(3) * s1[f] - R(2) * al;
-            }
-          }
+          const bool flat = lim_flat || (s2[f] - s1[f]) * (s1[f] - s0[f]) <= R(0);
+          const R da = ar - al;
+          const R a6 = R(6) * s1[f] - R(3) * (al + ar);
+          const bool peak_r = da * a6 > da * da;
+          const bool peak_l = da * a6 < -(da * da);
+          const R al_r = R(3) * s1[f] - R(2) * ar;
+          const R ar_l = R(3) * s1[f] - R(2) * al;
+          const R al2 = flat ? s1[f] : (peak_r ? al_r : al);
+          const R ar2 = flat ? s1[f] : ((!peak_r && peak_l) ? ar_l : ar);
+          al = al2;
+          ar = ar2;
         }
-        if (al != ar) {
-          c1l[f] = al;
-          c2l[f] = ar - al;
-          c3l[f] = R(6) * s1[f] - R(3) * (al + ar);
-        } else {
-          c1l[f] = al;
-        }
+        const bool pcm = !(al != ar);
+        c1l[f] = al;
+        c2l[f] = pcm ? R(0) : ar - al;
+        c3l[f] = pcm ? R(0) : R(6) * s1[f] - R(3) * (al + ar);
         st.asc[f] = as_n;
         st.alc[f] = al_n;
       }

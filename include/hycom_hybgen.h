@@ -195,7 +195,9 @@ int hycom_hybgen_set_kernel_path(int path);
  * FP64 with 4 <= kdm <= 63; otherwise 3); knob 1: levels loaded ahead by the
  * u/v walk, 0 or 2; knob 2: the u/v walk's new thicknesses kept in LDS (0)
  * or formed again from p as the walk needs them (1, no LDS); knob 3: the
- * regrid's window loaded one iteration ahead (1) or not (0).  Process-wide. */
+ * regrid's window loaded one iteration ahead (1) or not (0); knob 4: the
+ * u/v phase on a second stream of the library's, beside the p-column remap
+ * (1), or after it on the caller's stream (0).  Process-wide. */
 int hycom_hybgen_tune(int knob, int value);
 
 /* Measurement hooks (no reference counterpart; the reference's only timer

@@ -208,6 +208,30 @@ def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
 
 
+@pytest.mark.parametrize("hybmap,raflag", [(2, 0), (3, 1), (1, 0), (0, 1)])
+def test_concurrent_uv_phase(gpu, hybmap, raflag):
+    """The u/v phase on the library's second stream beside the p-column
+    remap (tuning knob 4) gives the oracle's bits, the RA branch included."""
+    P, w = _case("concurrent-uv", idm=90, jdm=57, kdm=41, hybmap=hybmap, ntracr=3,
+                 trcflg=[2, 0, 0])
+    _select(3, P)
+    lib = abi.load()
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=70 + hybmap))
+    if raflag:
+        rng = np.random.default_rng(hybmap)
+        d0.a["dpo"][...] = np.abs(d0.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d0.a["dp"].shape)))
+    want = _oracle_out(d0, raflag=raflag)
+    dd = abi.DeviceDomain(P).upload(d0)
+    abi.check(lib.hycom_hybgen_tune(4, 1))
+    try:
+        dd.hybgen(m=1, n=2, raflag=raflag)
+        got = dd.download()
+    finally:
+        abi.check(lib.hycom_hybgen_tune(4, 0))
+        abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    _assert_same(want, got, f"concurrent u/v hybmap={hybmap} raflag={raflag}")
+
+
 @pytest.mark.parametrize("kdm", [4, 9, 22, 31, 41, 47, 63])
 def test_tiled_remap(gpu, kdm):
     """The tiled PPM remap (remap variant 16) with 2, 3 and 4 layers per
