@@ -155,6 +155,12 @@ SIGNATURES = {
     "hycom_stokes_vertical_device": (ctypes.c_int, [_P(HybStokesParams), _P(HybStokesFields),
                                                     ctypes.c_int, ctypes.c_void_p]),
     "hycom_trc_remap": (ctypes.c_int, [_P(HybTrcParams), _P(HybTrcFields)]),
+    "hycom_za_read": (ctypes.c_int, [ctypes.c_char_p] + [ctypes.c_int32] * 5
+                      + [ctypes.c_void_p] * 4),
+    "hycom_za_read_device": (ctypes.c_int, [ctypes.c_char_p] + [ctypes.c_int32] * 5
+                             + [ctypes.c_void_p] * 5),
+    "hycom_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P(HybParams),
+                                          _P(HybFields), ctypes.c_void_p, ctypes.c_void_p]),
     "hycom_trc_remap_device": (ctypes.c_int, [_P(HybTrcParams), _P(HybTrcFields),
                                               ctypes.c_void_p]),
 }

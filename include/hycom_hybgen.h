@@ -419,6 +419,42 @@ typedef struct hyb_trc_fields {
 /* Host arrays in and out; returns when so holds the result. */
 int hycom_trc_remap(const hyb_trc_params* P, const hyb_trc_fields* F);
 
+/* ---------------------------------------------------------------------
+ * HYCOM .a/.b files (README.src.mod_za:46-60).  An .a file holds 2-D
+ * records of idm*jdm 32-bit IEEE reals, big-endian, in Fortran order, each
+ * padded with 2^100 to a multiple of 4096 words; 2^100 marks a data void.
+ * ------------------------------------------------------------------- */
+
+/* Reads records rec0..rec0+nrec-1 (1-based) of an .a file into
+ * out(1-nbdy:idm+nbdy, 1-nbdy:jdm+nbdy, nrec), as zaiord3 does
+ * (mod_za_sm.h:678-862, with zaio_endian machi_c.c:50-67): with mask NULL
+ * every interior point is written; with a mask (NI,NJ) only points where
+ * mask != 0, and a void there is an error (HYB_EINVAL).  Halo points are
+ * not written.  hmin[r], hmax[r] (may be NULL) get each record's range
+ * over the points read, voids left out, as zaiord returns it. */
+int hycom_za_read(const char* apath, int32_t idm, int32_t jdm, int32_t nbdy, int32_t rec0,
+                  int32_t nrec, const int32_t* mask, double* out, double* hmin, double* hmax);
+
+/* Same into device memory (out a device pointer; mask must be NULL): the
+ * records are staged as 32-bit words and widened on the device, ordered on
+ * `stream`; returns when the records are in place. */
+int hycom_za_read_device(const char* apath, int32_t idm, int32_t jdm, int32_t nbdy,
+                         int32_t rec0, int32_t nrec, const int32_t* mask, double* out,
+                         double* hmin, double* hmax, void* stream);
+
+/* The state hybgen reads, from a model restart pair (restart_in,
+ * mod_restart.F90:14-651): u, v, dp, temp, saln (2 time levels each),
+ * th3d (read from an old "RESTART:" file, else sig(temp,saln)-thbase at
+ * ip == 1 and 0 elsewhere, as restart_in forms it), ubavg, vbavg (3 levels)
+ * and, when P->ntracr > 0, the tracers, into host fields of F (NULL fields
+ * skipped).  Every record is found by its .b line (name in columns 1-8)
+ * and its range checked against that line to 1e-4, as restart_in3d does
+ * (HYB_EINVAL on a mismatch or a missing field).  P gives the extents,
+ * kdm, ntracr and thbase; F->ip is read for th3d.  nstep and dtime (may be
+ * NULL) get the header's values. */
+int hycom_restart_read(const char* apath, const char* bpath, const hyb_params* P,
+                       const hyb_fields* F, int32_t* nstep, double* dtime);
+
 /* Same on device pointers, ordered on `stream`; returns after enqueueing
  * (a column that would stop is left NaN, not reported).  Calls on one
  * device share a workspace: do not overlap them on different streams. */

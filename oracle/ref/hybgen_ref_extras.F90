@@ -116,4 +116,64 @@
       so = f1
       end subroutine href_plm_dpi_column
 
+! .a records through the reference's own array I/O (mod_za.F90 with
+! mod_za_sm.h): mode 1 writes h with zaiowr3 (real*4, 2^100 padding, byte
+! order swapped by zaio_endian under ENDIAN_IO), mode 0 reads it back with
+! zaiord3; both return each record's range.  zaiost allocates its buffer
+! for the extents of the first call, so every call must use those extents
+! (-2 otherwise).
+      integer(c_int) function href_za(mode, path, plen, idm_, jdm_, nrec, h, hmin, hmax) &
+          bind(C, name='href_za')
+      use mod_za
+      use hybgen_ref_driver, only: hyb_params_t, ensure_alloc
+      integer(c_int), value :: mode, plen, idm_, jdm_, nrec
+      character(kind=c_char), intent(in) :: path(plen)
+      real(c_double), intent(inout) :: h(idm_+2*nbdy, jdm_+2*nbdy, nrec)
+      real(c_double), intent(out) :: hmin(nrec), hmax(nrec)
+      type(hyb_params_t) :: prm
+      character(len=512) :: cfile
+      integer, allocatable :: msk(:,:)
+      real :: h8(idm_+2*nbdy, jdm_+2*nbdy, nrec), xmin(nrec), xmax(nrec)
+      integer :: k
+      logical, save :: ready = .false.
+      integer, save :: rid = -1, rjd = -1
+      href_za = -1
+      if (plen .gt. 512) return
+      cfile = ' '
+      do k= 1,plen
+        cfile(k:k) = path(k)
+      enddo
+      prm%idm = idm_
+      prm%jdm = jdm_
+      prm%kdm = 1
+      prm%ntracr = 0
+      call ensure_alloc(prm)
+      lp = 6
+      mnproc = 1
+      if (.not. ready) then
+        call zaiost
+        ready = .true.
+        rid = idm_
+        rjd = jdm_
+      elseif (rid .ne. idm_ .or. rjd .ne. jdm_) then
+        href_za = -2
+        return
+      endif
+      allocate(msk(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy))
+      msk = 1
+      h8 = h
+      if (mode .eq. 1) then
+        call zaiopf(cfile(1:plen), 'new', 77)
+        call zaiowr3(h8, nrec, msk, .false., xmin, xmax, 77, .true.)
+      else
+        call zaiopf(cfile(1:plen), 'old', 77)
+        call zaiord3(h8, nrec, msk, .false., xmin, xmax, 77)
+      endif
+      call zaiocl(77)
+      h = h8
+      hmin = xmin
+      hmax = xmax
+      href_za = 0
+      end function href_za
+
       end module hybgen_ref_extras

@@ -69,6 +69,10 @@ def ref(denfix=False):
             ctypes.c_int, ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
             _P(ctypes.c_double), _P(ctypes.c_double), ctypes.c_double, _P(ctypes.c_double)]
         lib.href_plm_dpi_column.restype = None
+        lib.href_za.restype = ctypes.c_int
+        lib.href_za.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, _P(ctypes.c_double),
+                                _P(ctypes.c_double), _P(ctypes.c_double)]
         _refs[denfix] = lib
     return _refs[denfix]
 
