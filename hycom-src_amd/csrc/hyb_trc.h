@@ -49,55 +49,84 @@ HYB_HDI void trc_slopes(const KTrc& Q, int64_t c0, int f0, double* mn, double* m
 #pragma unroll
   for (int f = 0; f < NF; f++) a[f] = Q.si + c0 + S * (int64_t)ki * (f0 + f);
   auto A = [&](int f, int l) -> double { return a[f][(int64_t)(l - 1) * S]; };
-  auto PT = [&](int l) -> double {  // pit(l) = max(pi(l+1)-pi(l), thin) (969-971)
-    return dmax(p[(int64_t)l * S] - p[(int64_t)(l - 1) * S], kTrcThin);
-  };
   double* s = Q.slope + c0;
   auto SL = [&](int f, int l) -> double& { return s[S * ((int64_t)(l - 1) * kTrcChunk + f)]; };
-  // raw slopes of zone j for every field
-  auto raw = [&](int j, double* r) {
-#pragma unroll
-    for (int f = 0; f < NF; f++) r[f] = 0.0;
-    if (j <= 1 || j >= ki) return;
-    const double pm = PT(j - 1), p0 = PT(j), pp = PT(j + 1);
-    const double rl = 2.0 * p0 / (pm + p0);
-    const double rr = 2.0 * p0 / (p0 + pp);
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      const double sl = rl * (A(f, j) - A(f, j - 1));
-      const double sr = rr * (A(f, j + 1) - A(f, j));
-      if (sl * sr > 0.0) r[f] = copysign(dmin(fabs(sl), fabs(sr)), sl);
-    }
+  // windows a(j-1..j+2) of every field and pit(j-1..j+2), pit(l) =
+  // max(pi(l+1)-pi(l), thin) (969-971); pn = pi(l+1) of the last pit formed
+  double am[NF], a0[NF], ap[NF], aq[NF];
+  double pn = p[0];
+  auto next_pt = [&](int l) -> double {  // pit(l), l increasing by one from 1
+    const double pl1 = p[(int64_t)l * S];
+    const double t = dmax(pl1 - pn, kTrcThin);
+    pn = pl1;
+    return t;
   };
+  const double tm0 = next_pt(1);
+  const double t00 = (ki >= 2) ? next_pt(2) : 0.0;
+  const double tp0 = (ki >= 3) ? next_pt(3) : 0.0;
+  const double tq0 = (ki >= 4) ? next_pt(4) : 0.0;
+  double tm = tm0, t0 = t00, tp = tp0, tq = tq0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    double lo = A(f, 1), hi = lo;
-    for (int l = 2; l <= ki; l++) {
-      const double x = A(f, l);
-      lo = dmin(lo, x);
-      hi = dmax(hi, x);
-    }
+    am[f] = A(f, 1);
+    a0[f] = (ki >= 2) ? A(f, 2) : 0.0;
+    ap[f] = (ki >= 3) ? A(f, 3) : 0.0;
+    aq[f] = (ki >= 4) ? A(f, 4) : 0.0;
+    double lo = am[f], hi = am[f];  // the range over all ki levels (any order: min/max are exact)
+    if (ki >= 2) { lo = dmin(lo, a0[f]); hi = dmax(hi, a0[f]); }
+    if (ki >= 3) { lo = dmin(lo, ap[f]); hi = dmax(hi, ap[f]); }
+    if (ki >= 4) { lo = dmin(lo, aq[f]); hi = dmax(hi, aq[f]); }
     mn[f] = lo;
     mx[f] = hi;
     SL(f, 1) = 0.0;
   }
-  double sm[NF], rj[NF], rn[NF];
+  // raw slope (Campbell eq. 15) of a zone from its window
+  auto raw = [&](double xm, double x0, double xp, double sm_, double s0_, double sp_) -> double {
+    const double rl = 2.0 * s0_ / (sm_ + s0_);
+    const double rr = 2.0 * s0_ / (s0_ + sp_);
+    const double sl = rl * (x0 - xm);
+    const double sr = rr * (xp - x0);
+    return (sl * sr > 0.0) ? copysign(dmin(fabs(sl), fabs(sr)), sl) : 0.0;
+  };
+  double sm[NF], rj[NF];
 #pragma unroll
-  for (int f = 0; f < NF; f++) sm[f] = 0.0;  // s(j-1), changed
-  raw(2, rj);                                // raw s(j)
+  for (int f = 0; f < NF; f++) {
+    sm[f] = 0.0;                                                     // s(j-1), changed
+    rj[f] = (ki >= 3) ? raw(am[f], a0[f], ap[f], tm, t0, tp) : 0.0;  // raw s(2)
+  }
   for (int j = 2; j <= ki - 1; j++) {
-    raw(j + 1, rn);                          // raw s(j+1)
+    const bool has_n = j + 1 <= ki - 1;  // zone j+1 is an inner zone
 #pragma unroll
     for (int f = 0; f < NF; f++) {
+      const double rn = has_n ? raw(a0[f], ap[f], aq[f], t0, tp, tq) : 0.0;  // raw s(j+1)
       double sj = rj[f];
       if (sj != 0.0) {
-        const double dl = -0.5 * (sj + sm[f]) + A(f, j) - A(f, j - 1);
-        const double dr = -0.5 * (rn[f] + sj) + A(f, j + 1) - A(f, j);
+        const double dl = -0.5 * (sj + sm[f]) + a0[f] - am[f];
+        const double dr = -0.5 * (rn + sj) + ap[f] - a0[f];
         sj = sj + 2.0 * copysign(dmin(fabs(dl), fabs(dr)), dl);
       }
       SL(f, j) = sj;
       sm[f] = sj;
-      rj[f] = rn[f];
+      rj[f] = rn;
+    }
+    // slide the windows by one zone
+    const bool more = j + 3 <= ki;
+    tm = t0;
+    t0 = tp;
+    tp = tq;
+    tq = more ? next_pt(j + 3) : 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      am[f] = a0[f];
+      a0[f] = ap[f];
+      ap[f] = aq[f];
+      if (more) {
+        aq[f] = A(f, j + 3);
+        mn[f] = dmin(mn[f], aq[f]);
+        mx[f] = dmax(mx[f], aq[f]);
+      } else {
+        aq[f] = 0.0;
+      }
     }
   }
 #pragma unroll
@@ -146,13 +175,14 @@ HYB_HDI bool trc_remap_fields(const KTrc& Q, int64_t c0, int f0) {
       const double dz = zb - zt;
 #pragma unroll
       for (int f = 0; f < NF; f++) sok[f] = 0.0;
+      double pl1 = PI(lf);  // pi(l), carried from the iteration before
       for (int l = lf; l <= ki; l++) {
-        const double pl = PI(l);
+        const double pl = pl1;
         if (pl > zb) {
           lf = l - 1;
           break;
         }
-        const double pl1 = PI(l + 1);
+        pl1 = PI(l + 1);
         if (pl >= zt && pl1 <= zb) {
           const double q = dmax(pl1 - pl, thin) / dz;
 #pragma unroll
