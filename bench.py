@@ -234,7 +234,7 @@ def stokes_leg(lib, dd, P, sea, args, stream):
     out = {"value": sea * P.kdm / (ms * 1e-3), "unit": "column-layers/s", "ms_per_call": ms,
            "nsdzi": nz, "kdm": P.kdm, "bytes_per_sea_column": bpc,
            "achieved_GBs": sea * bpc / (ms * 1e-3) / 1e9,
-           "kernels": "k_stokes_blend + k_stokes_walk (walk_core PLM, 2 fields)"}
+           "kernels": "k_stokes_walk (walk_core PLM, 2 fields, snapshots blended as read)"}
     print(f"[bench] stokes {ms:.2f} ms/call", file=sys.stderr, flush=True)
     del usdz, vsdz, usdp, vsdp
     torch.cuda.empty_cache()

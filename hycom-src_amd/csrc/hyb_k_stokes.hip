@@ -1,6 +1,6 @@
 // hyb_k_stokes.hip -- Stokes drift onto the model layers (hyb_stokes.h):
-// the blend of the two snapshots, then the PLM column walk.  Lanes along
-// i on 128-byte-aligned rows, blockIdx.y = j.
+// the PLM column walk, blending the two snapshots as it reads them.  Lanes
+// along i on 128-byte-aligned rows, blockIdx.y = j.
 #include "hyb_launch.h"
 #include "hyb_stokes.h"
 
@@ -8,15 +8,6 @@ namespace hyb {
 
 HYB_HD int stokes_i0(const KStokes& Q, int j) {
   return 1 - (int)(((int64_t)Q.nb + Q.NI * (int64_t)(j - 1 + Q.nb)) & 15);
-}
-
-__global__ void __launch_bounds__(kBlock) k_stokes_blend(KStokes Q) {
-  const int j = blockIdx.y + 1;
-  const int i = stokes_i0(Q, j) + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 1 || i > Q.ii) return;
-  const int64_t c0 = stokes_ij(Q, i, j);
-  if (Q.ip[c0] == 0) return;
-  for (int k = 1; k <= Q.nsdzi; k++) stokes_blend(Q, c0, k);
 }
 
 __global__ void __launch_bounds__(kWave) k_stokes_walk(KStokes Q) {
@@ -29,9 +20,7 @@ __global__ void __launch_bounds__(kWave) k_stokes_walk(KStokes Q) {
 }
 
 hipError_t launch_stokes(const KStokes& Q, hipStream_t s) {
-  dim3 gb((Q.ii + kAlignPad + kBlock - 1) / kBlock, Q.jj);
   dim3 gw((Q.ii + kAlignPad + kWave - 1) / kWave, Q.jj);
-  hipLaunchKernelGGL(k_stokes_blend, gb, dim3(kBlock), 0, s, Q);
   hipLaunchKernelGGL(k_stokes_walk, gw, dim3(kWave), 0, s, Q);
   return hipGetLastError();
 }

@@ -37,8 +37,7 @@ int stokes_check(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n) 
   return 0;
 }
 
-KStokes kstokes_of(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n,
-                   double* blend) {
+KStokes kstokes_of(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n) {
   KStokes Q;
   memset(&Q, 0, sizeof Q);
   Q.ii = P->ii;
@@ -59,25 +58,7 @@ KStokes kstokes_of(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n
   Q.ip = F->ip;
   Q.usdp = F->usdp;
   Q.vsdp = F->vsdp;
-  Q.blend = blend;
   return Q;
-}
-
-// The blend workspace (2*nsdzi levels), grown on demand and kept.
-std::mutex g_blend_mu;
-double* g_blend = nullptr;
-size_t g_blend_n = 0;
-
-int blend_for(size_t n, double** out) {
-  if (n > g_blend_n) {
-    if (g_blend) HIPCHK(hipFree(g_blend));
-    g_blend = nullptr;
-    g_blend_n = 0;
-    HIPCHK(hipMalloc(&g_blend, n * sizeof(double)));
-    g_blend_n = n;
-  }
-  *out = g_blend;
-  return 0;
 }
 
 // device mirrors of the host call, kept between calls of one shape
@@ -111,11 +92,7 @@ int hycom_stokes_vertical_device(const hyb_stokes_params* P, const hyb_stokes_fi
   int rc = stokes_check(P, F, n);
   if (rc) return rc;
   if ((rc = device_ready())) return rc;
-  std::lock_guard<std::mutex> lk(g_blend_mu);
-  const int64_t NIJ = ((int64_t)P->idm + 2 * P->nbdy) * ((int64_t)P->jdm + 2 * P->nbdy);
-  double* blend = nullptr;
-  if ((rc = blend_for((size_t)NIJ * 2 * P->nsdzi, &blend))) return rc;
-  KStokes Q = kstokes_of(P, F, n, blend);
+  KStokes Q = kstokes_of(P, F, n);
   HIPCHK(launch_stokes(Q, (hipStream_t)stream));
   return 0;
 }

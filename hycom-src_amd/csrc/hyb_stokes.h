@@ -1,9 +1,8 @@
 // hyb_stokes.h -- Stokes drift onto the model layers (mod_stokes.F90:650-705,
-// stokes_vertical_j), shared by the host checks and the gfx950 kernels.
-// One sea p-column per lane.  The blended snapshots are formed first
-// (stokes_blend, into a workspace of 2*nsdzi levels); the column walk is
-// walk_core's PLM remap from the nsdzi fixed layers onto the kdm model
-// layers.
+// stokes_vertical_j), shared by the host checks and the gfx950 kernel.
+// One sea p-column per lane: walk_core's PLM remap from the nsdzi fixed
+// layers onto the kdm model layers, reading both snapshots of a level and
+// blending them when the level enters the walk's window.
 #pragma once
 
 #include "hyb_stream.h"
@@ -18,7 +17,6 @@ struct KStokes {
   const double *usdz, *vsdz, *dp, *p;
   const int32_t* ip;
   double *usdp, *vsdp;
-  double* blend;  // (NIJ, nsdzi, 2): the blended u then v profiles
 };
 
 HYB_HD int64_t stokes_ij(const KStokes& Q, int i, int j) {
@@ -65,13 +63,26 @@ struct PoStokes {
   }
 };
 
-// s1d of one level (mod_stokes.F90:676-679) into the workspace
-HYB_HDI void stokes_blend(const KStokes& Q, int64_t c0, int k) {
-  const int64_t a = c0 + Q.NIJ * (int64_t)(2 * (k - 1));
-  const int64_t o = c0 + Q.NIJ * (int64_t)(k - 1);
-  Q.blend[o] = Q.ws0 * Q.usdz[a] + Q.ws1 * Q.usdz[a + Q.NIJ];
-  Q.blend[o + Q.NIJ * (int64_t)Q.nsdzi] = Q.ws0 * Q.vsdz[a] + Q.ws1 * Q.vsdz[a + Q.NIJ];
-}
+// s1d(k) = ws0*usdz(i,j,1,k) + ws1*usdz(i,j,2,k) (mod_stokes.F90:676-679):
+// both words are loaded when the walk asks for level k, and blended when
+// the value is used
+struct StokesBlend {
+  double x, y, w0, w1;
+  HYB_HD operator double() const { return w0 * x + w1 * y; }
+};
+struct StokesRow {  // one field of one column; level stride 2S, snapshot 2 at +S
+  const double* a;
+  int64_t S;
+  double w0, w1;
+  HYB_HD StokesBlend operator[](int64_t i) const {
+    const int64_t o = 2 * i;  // i = (k-1)*S as walk_core forms it
+    return StokesBlend{a[o], a[o + S], w0, w1};
+  }
+};
+struct StokesFields {
+  StokesRow row[2];
+  HYB_HD const StokesRow& operator[](int f) const { return row[f]; }
+};
 
 // hybgen_plm_coefs on dpi and hybgen_plm_remap of the two blended fields
 // (mod_stokes.F90:686-694), results to usdp, vsdp (695-698)
@@ -81,7 +92,8 @@ HYB_HDI void stokes_column(const KStokes& Q, int64_t c0) {
   const double pbot = Q.p[c0 + S * (int64_t)Q.kk];
   PiStokes pi{Q.sdzi, pbot};
   PoStokes po(Q.dp + c0 + S * (int64_t)Q.kk * (Q.n - 1), S, pbot, Q.kk);
-  double* fp[2] = {Q.blend + c0, Q.blend + c0 + S * (int64_t)Q.nsdzi};
+  const StokesFields fp{{StokesRow{Q.usdz + c0, S, Q.ws0, Q.ws1},
+                         StokesRow{Q.vsdz + c0, S, Q.ws0, Q.ws1}}};
   double* op[2] = {Q.usdp + c0, Q.vsdp + c0};
   FieldSink<2> sink{op, 2, nullptr, 0.0, S};
   const double zx = dmin(Q.sdzi[Q.nsdzi - 1], pbot);

@@ -1054,14 +1054,17 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
 // interfaces) are loaded ahead of the window, to cover memory latency.
 // K old layers, KO new ones (KO = 0: K).  CDPI: the PLM slopes are taken
 // on dpi = max(dprs, thin) instead of dprs (mod_stokes.F90:683-684 passes
-// dpi to hybgen_plm_coefs; hybgen.F90:1535 passes dprs).
+// dpi to hybgen_plm_coefs; hybgen.F90:1535 passes dprs).  fp[f][i] is a
+// double, or a value that converts to one when the level enters the window
+// (a blend of two stored words, so the loads stay outstanding until then).
 // ---------------------------------------------------------------------
 template <int METHOD, int NF, int PD, bool FAST = false, bool CDPI = false, class PI, class PO,
-          class SINK>
-HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, double* const* fp,
+          class SINK, class FP>
+HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const FP& fp,
                       int nf, int kp, uint64_t lcm, int64_t S, int KO = 0) {
   constexpr int PQ = PD > 0 ? PD : 1;
   const int KN = KO > 0 ? KO : K;
+  using RawT = typename std::decay<decltype(fp[0][0])>::type;
   using CR = typename std::conditional<FAST, float, double>::type;
   const double thin = kDpthin;
   const double dsmll = 1.0e-8;
@@ -1072,7 +1075,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
   double smin[NF], smax[NF], skp[NF], sfirst[NF];
   double acc[NF], ov[NF], prev[NF];
   CoefState<CR, NF> cst;                       // reconstruction state
-  double sq[NF][PQ];                           // s(l+3..l+2+PD), loaded ahead
+  RawT sq[NF][PQ];                             // s(l+3..l+2+PD), loaded ahead
   double prq[PQ];                              // raw(l+3..l+2+PD) of pi
   int xl = 4;                                  // next level to load
   // s(x) with the massless fill (972-996) and the running min/max
@@ -1092,12 +1095,12 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
     return v;
   };
   auto load = [&](int f, int x) -> double {
-    return fold(f, x, (x <= kp) ? fp[f][(int64_t)(x - 1) * S] : 0.0);
+    return fold(f, x, (x <= kp) ? static_cast<double>(fp[f][(int64_t)(x - 1) * S]) : 0.0);
   };
   // the stored value of level min(x, kp), loaded without a use: with
   // PD > 0 it is folded PD layer iterations later, so the load has that
   // long to arrive (its value is discarded below kp)
-  auto raw = [&](int f, int x) -> double {
+  auto raw = [&](int f, int x) -> RawT {
     const int xc = x < kp ? x : kp;
     return fp[f][(int64_t)(xc - 1) * S];
   };
@@ -1114,7 +1117,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
       if (K >= 3) s3[f] = load(f, 3);
       sfirst[f] = s1[f];
 #pragma unroll
-      for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? raw(f, 4 + q) : 0.0;
+      for (int q = 0; q < PD; q++) sq[f][q] = (4 + q <= K) ? raw(f, 4 + q) : RawT{};
       if (METHOD == 2) cst.alc[f] = (CR)s1[f];  // al(1) = s(1)
       if (METHOD == 3) {                        // first sweep of layer 1
         cst.wc1[f] = (CR)s1[f];
@@ -1420,7 +1423,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
                 else if (mid == l + 1) o = s2[f];
                 else if (mid == l + 2) o = s3[f];
                 else if (kp <= l + 2) o = skp[f];
-                else o = fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S];
+                else o = static_cast<double>(fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S]);
                 ov[f] = o;
                 if (METHOD == 0) {
                   acc[f] = dpt * (s1[f] - o);
@@ -1471,7 +1474,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, doubl
         s3[f] = (f < nf && l + 3 <= K) ? fold(f, l + 3, sq[f][0]) : 0.0;
 #pragma unroll
         for (int q = 0; q < PD - 1; q++) sq[f][q] = sq[f][q + 1];
-        sq[f][PQ - 1] = (f < nf && xl <= K) ? raw(f, xl) : 0.0;
+        sq[f][PQ - 1] = (f < nf && xl <= K) ? raw(f, xl) : RawT{};
       }
     }
     xl++;

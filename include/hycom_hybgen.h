@@ -381,8 +381,7 @@ typedef struct hyb_stokes_fields {
 int hycom_stokes_vertical(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n);
 
 /* Same on device pointers (sdzi stays host memory), ordered on `stream`;
- * returns after enqueueing.  Calls on one device share a workspace: do not
- * overlap them on different streams. */
+ * returns after enqueueing.  Allocates nothing. */
 int hycom_stokes_vertical_device(const hyb_stokes_params* P, const hyb_stokes_fields* F, int n,
                                  void* stream);
 

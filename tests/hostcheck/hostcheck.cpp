@@ -456,14 +456,11 @@ extern "C" int hostcheck_stokes(const hyb_stokes_params* P, const hyb_stokes_fie
   Q.ip = F->ip;
   Q.usdp = F->usdp;
   Q.vsdp = F->vsdp;
-  std::vector<double> blend((size_t)Q.NIJ * 2 * Q.nsdzi, 0.0);
-  Q.blend = blend.data();
 #pragma omp parallel for schedule(dynamic, 1)
   for (int j = 1; j <= Q.jj; j++)
     for (int i = 1; i <= Q.ii; i++) {
       const int64_t c0 = stokes_ij(Q, i, j);
       if (!Q.ip[c0]) continue;
-      for (int k = 1; k <= Q.nsdzi; k++) stokes_blend(Q, c0, k);
       stokes_column<2>(Q, c0);
     }
   return 0;
