@@ -10,6 +10,56 @@
 
 namespace hyb {
 
+// A pointer read from memory is a generic (flat) address to the compiler,
+// so every access through it becomes a flat instruction, which counts
+// against both the vector-memory and the LDS/scalar wait counters.  A
+// pointer loaded from the constant address space is taken to be global
+// (only the host fills that memory, and it can only name global memory),
+// so the domain's field and workspace pointers are read through it, one
+// member at a time: the accesses become global loads and stores, as in the
+// single-domain kernels, whose pointers arrive as arguments.
+template <class T>
+using cview = __attribute__((address_space(4))) const T;
+__device__ __forceinline__ View gview(const View* a, int d) {
+  const cview<View>* q = (const cview<View>*)(a + d);
+  View v;
+  v.dp = q->dp;
+  v.dpo = q->dpo;
+  v.dpu = q->dpu;
+  v.dpv = q->dpv;
+  v.temp = q->temp;
+  v.saln = q->saln;
+  v.th3d = q->th3d;
+  v.u = q->u;
+  v.v = q->v;
+  v.tracer = q->tracer;
+  v.p = q->p;
+  v.pu = q->pu;
+  v.pv = q->pv;
+  v.theta = q->theta;
+  v.depths = q->depths;
+  v.topiso = q->topiso;
+  v.depthu = q->depthu;
+  v.depthv = q->depthv;
+  v.ubavg = q->ubavg;
+  v.vbavg = q->vbavg;
+  v.ip = q->ip;
+  v.iu = q->iu;
+  v.iv = q->iv;
+  v.NI = q->NI;
+  v.NIJ = q->NIJ;
+  v.nb = q->nb;
+  v.K = q->K;
+  v.rec_fixlay = q->rec_fixlay;
+  v.rec_kp = q->rec_kp;
+  v.rec_ld = q->rec_ld;
+  return v;
+}
+__device__ __forceinline__ Scratch gscratch(const Scratch* a, int d) {
+  const cview<Scratch>* q = (const cview<Scratch>*)(a + d);
+  return Scratch{q->pres, q->kp, q->lcm, q->uvf};
+}
+
 // largest g with prefix[g] <= b (prefix[0] = 0, prefix[ng] = total)
 __device__ __forceinline__ int group_of(const int32_t* prefix, int ng, int b) {
   int lo = 0, hi = ng;
@@ -33,7 +83,7 @@ __device__ __forceinline__ RowPos row_pos(const BatchDevice& B, const BatchGroup
   const int local = b - G.prefix[g];
   const int nbi = (B.P[d].ii + kAlignPad + bw - 1) / bw;
   const int j = local / nbi + 1;
-  return RowPos{d, j, i_aligned0(B.V[d], j) + (local % nbi) * bw};
+  return RowPos{d, j, i_aligned0(gview(B.V, d), j) + (local % nbi) * bw};
 }
 
 template <int BW>
@@ -41,11 +91,11 @@ __global__ void __launch_bounds__(BW) __attribute__((amdgpu_waves_per_eu(3)))
 kb_regrid(BatchDevice B, BatchGroup G, int n) {
   const RowPos r = row_pos(B, G, BW, blockIdx.x);
   const KParams& P = B.P[r.d];
-  const View& V = B.V[r.d];
+  const View V = gview(B.V, r.d);
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, r.j)] == 0) return;
-  regrid_stream<true>(P, V, B.W[r.d], i, r.j, n);
+  regrid_stream<true>(P, V, gscratch(B.W, r.d), i, r.j, n);
 }
 
 // CW columns per workgroup; each lane's LDS column has stride CW doubles
@@ -54,8 +104,8 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   extern __shared__ double lds[];
   const RowPos r = row_pos(B, G, CW, blockIdx.x);
   const KParams& P = B.P[r.d];
-  const View& V = B.V[r.d];
-  const Scratch& W = B.W[r.d];
+  const View V = gview(B.V, r.d);
+  const Scratch W = gscratch(B.W, r.d);
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, r.j);
@@ -72,8 +122,8 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
 __global__ void __launch_bounds__(kWave) kb_remap_fb(BatchDevice B, BatchGroup G, int n) {
   const RowPos r = row_pos(B, G, kWave, blockIdx.x);
   const KParams& P = B.P[r.d];
-  const View& V = B.V[r.d];
-  const Scratch& W = B.W[r.d];
+  const View V = gview(B.V, r.d);
+  const Scratch W = gscratch(B.W, r.d);
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, r.j);
@@ -93,7 +143,7 @@ __global__ void __launch_bounds__(kBlock) kb_halo_j(BatchDevice B, BatchGroup G)
   const int k = local / nbi + 2;
   const int i = (local % nbi) * kBlock + threadIdx.x + 1;
   if (i > P.ii) return;
-  p_halo_jrow(P, B.V[d], i, k);
+  p_halo_jrow(P, gview(B.V, d), i, k);
 }
 __global__ void __launch_bounds__(kBlock) kb_halo_i(BatchDevice B, BatchGroup G) {
   const int g = group_of(G.prefix, G.ndom, blockIdx.x);
@@ -104,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) kb_halo_i(BatchDevice B, BatchGroup G)
   const int k = local / nbj + 2;
   const int j = (local % nbj) * kBlock + threadIdx.x;
   if (j > P.jj + 1) return;
-  p_halo_icol(P, B.V[d], j, k);
+  p_halo_icol(P, gview(B.V, d), j, k);
 }
 
 // u- and v-points: the first half of a domain's blocks are u, then v
@@ -114,7 +164,7 @@ __global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int 
   const int g = group_of(G.prefix, G.ndom, blockIdx.x);
   const int d = G.dom[g];
   const KParams& P = B.P[d];
-  const View& V = B.V[d];
+  const View V = gview(B.V, d);
   const int nbi = (P.ii + kAlignPad + kWave - 1) / kWave;
   int local = blockIdx.x - G.prefix[g];
   const bool isv = local >= P.jj * nbi;
@@ -127,14 +177,14 @@ __global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int 
   if (msk[c0] == 0) return;
   LaneRow dcol{lds + threadIdx.x, kWave};  // unused: new thicknesses formed again from p
   const bool ok = uv_stream<METHOD, 2, MIXED, true>(P, V, i, j, nl, isv, dcol);
-  B.W[d].uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
+  gscratch(B.W, d).uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
 }
 
 __global__ void __launch_bounds__(kWave) kb_uv_fb(BatchDevice B, BatchGroup G, int nl) {
   const int g = group_of(G.prefix, G.ndom, blockIdx.x);
   const int d = G.dom[g];
   const KParams& P = B.P[d];
-  const View& V = B.V[d];
+  const View V = gview(B.V, d);
   const int nbi = (P.ii + kAlignPad + kWave - 1) / kWave;
   int local = blockIdx.x - G.prefix[g];
   const bool isv = local >= P.jj * nbi;
@@ -145,7 +195,7 @@ __global__ void __launch_bounds__(kWave) kb_uv_fb(BatchDevice B, BatchGroup G, i
   const int64_t c0 = ij_of(V, i, j);
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
-  if (B.W[d].uvf[c0 + (isv ? V.NIJ : 0)] == 0) return;
+  if (gscratch(B.W, d).uvf[c0 + (isv ? V.NIJ : 0)] == 0) return;
   point_c<64>(P, V, i, j, nl, isv);
 }
 
