@@ -104,6 +104,85 @@ struct BLine {
   double hmin = 0.0, hmax = 0.0;
 };
 
+// A restart pair: the .b header and one line per .a record (restart_in,
+// mod_restart.F90:45-77, and restart_in3d, 581-617).
+struct RestartPair {
+  const char* apath = nullptr;
+  const char* bpath = nullptr;
+  const hyb_params* P = nullptr;
+  bool old_format = false;
+  int32_t nstep = 0;
+  double dtime = 0.0;
+  int64_t NI = 0, NIJ = 0;
+  std::vector<BLine> rec;
+
+  int open_(const char* a, const char* b, const hyb_params* p) {
+    apath = a;
+    bpath = b;
+    P = p;
+    FILE* fb = fopen(bpath, "r");
+    if (!fb) return fail(HYB_EINVAL, "cannot open %s", bpath);
+    std::vector<std::string> lines;
+    char line[512];
+    while (fgets(line, sizeof line, fb)) lines.emplace_back(line);
+    fclose(fb);
+    if (lines.size() < 2) return fail(HYB_EINVAL, "%s: no restart header", bpath);
+    // header: the type, then 'nstep,dtime = ...'
+    if (lines[0].compare(0, 9, "RESTART: ") == 0) old_format = true;
+    else if (lines[0].compare(0, 9, "RESTART2:") == 0) old_format = false;
+    else return fail(HYB_EINVAL, "%s: unknown restart type", bpath);
+    const size_t q = lines[1].find('=');
+    long ns = 0;
+    if (q == std::string::npos || sscanf(lines[1].c_str() + q + 1, "%ld %lf", &ns, &dtime) != 2)
+      return fail(HYB_EINVAL, "%s: bad nstep,dtime line", bpath);
+    nstep = (int32_t)ns;
+    for (size_t k = 2; k < lines.size(); k++) {
+      BLine bl;
+      std::string& L = lines[k];
+      while (!L.empty() && (L.back() == '\n' || L.back() == '\r')) L.pop_back();
+      bl.name = L.substr(0, 8);
+      bl.name.resize(8, ' ');
+      const size_t e = L.find('=');
+      int layer = 0, level = 0;
+      if (e != std::string::npos)
+        sscanf(L.c_str() + e + 1, "%d %d %lf %lf", &layer, &level, &bl.hmin, &bl.hmax);
+      rec.push_back(bl);
+    }
+    NI = (int64_t)P->idm + 2 * P->nbdy;
+    NIJ = NI * ((int64_t)P->jdm + 2 * P->nbdy);
+    return 0;
+  }
+
+  // the first n records named `name`, which must be consecutive, into dst
+  int field(const char* name, int n, double* dst) const {
+    if (!dst) return 0;
+    std::string nm(name);
+    nm.resize(8, ' ');
+    size_t first = rec.size();
+    for (size_t r = 0; r < rec.size(); r++)
+      if (rec[r].name == nm) {
+        first = r;
+        break;
+      }
+    if (first + n > rec.size()) return fail(HYB_EINVAL, "%s: no %d records '%s'", bpath, n, name);
+    for (int k = 0; k < n; k++)
+      if (rec[first + k].name != nm)
+        return fail(HYB_EINVAL, "%s: record %d is '%s', expected '%s'", bpath, (int)(first + k + 1),
+                    rec[first + k].name.c_str(), name);
+    std::vector<double> mn(n, 0.0), mx(n, 0.0);
+    int rc = hycom_za_read(apath, P->idm, P->jdm, P->nbdy, (int)first + 1, n, nullptr, dst,
+                           mn.data(), mx.data());
+    if (rc) return rc;
+    for (int k = 0; k < n; k++) {  // restart_in3d's consistency check (604-616)
+      const BLine& b = rec[first + k];
+      if (fabs(mn[k] - b.hmin) > fabs(b.hmin) * 1.e-4 || fabs(mx[k] - b.hmax) > fabs(b.hmax) * 1.e-4)
+        return fail(HYB_EINVAL, "%s and %s not consistent at record %d ('%s')", apath, bpath,
+                    (int)(first + k + 1), name);
+    }
+    return 0;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -183,72 +262,14 @@ int hycom_restart_read(const char* apath, const char* bpath, const hyb_params* P
   if (!apath || !bpath || !P || !F) return fail(HYB_EINVAL, "null argument");
   const int kdm = P->kdm, ntr = P->ntracr;
   if (kdm < 1 || kdm > HYB_KMAX) return fail(HYB_EINVAL, "kdm=%d", kdm);
-  FILE* fb = fopen(bpath, "r");
-  if (!fb) return fail(HYB_EINVAL, "cannot open %s", bpath);
-  std::vector<std::string> lines;
-  char line[512];
-  while (fgets(line, sizeof line, fb)) lines.emplace_back(line);
-  fclose(fb);
-  if (lines.size() < 2) return fail(HYB_EINVAL, "%s: no restart header", bpath);
-  // header (mod_restart.F90:45-77): the type, then 'nstep,dtime = ...'
-  bool old_format;
-  if (lines[0].compare(0, 9, "RESTART: ") == 0) old_format = true;
-  else if (lines[0].compare(0, 9, "RESTART2:") == 0) old_format = false;
-  else return fail(HYB_EINVAL, "%s: unknown restart type", bpath);
-  {
-    const size_t q = lines[1].find('=');
-    long ns = 0;
-    double dt = 0.0;
-    if (q == std::string::npos || sscanf(lines[1].c_str() + q + 1, "%ld %lf", &ns, &dt) != 2)
-      return fail(HYB_EINVAL, "%s: bad nstep,dtime line", bpath);
-    if (nstep) *nstep = (int32_t)ns;
-    if (dtime) *dtime = dt;
-  }
-  // one .b line per .a record (restart_in3d, 581-617)
-  std::vector<BLine> rec;
-  for (size_t k = 2; k < lines.size(); k++) {
-    BLine b;
-    std::string& L = lines[k];
-    while (!L.empty() && (L.back() == '\n' || L.back() == '\r')) L.pop_back();
-    b.name = L.substr(0, 8);
-    b.name.resize(8, ' ');
-    const size_t q = L.find('=');
-    int layer = 0, level = 0;
-    if (q != std::string::npos)
-      sscanf(L.c_str() + q + 1, "%d %d %lf %lf", &layer, &level, &b.hmin, &b.hmax);
-    rec.push_back(b);
-  }
-  const int64_t NI = (int64_t)P->idm + 2 * P->nbdy, NIJ = NI * ((int64_t)P->jdm + 2 * P->nbdy);
-  std::vector<double> mn, mx;
-  // the first n records named `name`, which must be consecutive, into dst
-  auto field = [&](const char* name, int n, double* dst) -> int {
-    if (!dst) return 0;
-    std::string nm(name);
-    nm.resize(8, ' ');
-    size_t first = rec.size();
-    for (size_t r = 0; r < rec.size(); r++)
-      if (rec[r].name == nm) {
-        first = r;
-        break;
-      }
-    if (first + n > rec.size()) return fail(HYB_EINVAL, "%s: no %d records '%s'", bpath, n, name);
-    for (int k = 0; k < n; k++)
-      if (rec[first + k].name != nm)
-        return fail(HYB_EINVAL, "%s: record %d is '%s', expected '%s'", bpath, (int)(first + k + 1),
-                    rec[first + k].name.c_str(), name);
-    mn.assign(n, 0.0);
-    mx.assign(n, 0.0);
-    int rc = hycom_za_read(apath, P->idm, P->jdm, P->nbdy, (int)first + 1, n, nullptr, dst,
-                           mn.data(), mx.data());
-    if (rc) return rc;
-    for (int k = 0; k < n; k++) {  // restart_in3d's consistency check (604-616)
-      const BLine& b = rec[first + k];
-      if (fabs(mn[k] - b.hmin) > fabs(b.hmin) * 1.e-4 || fabs(mx[k] - b.hmax) > fabs(b.hmax) * 1.e-4)
-        return fail(HYB_EINVAL, "%s and %s not consistent at record %d ('%s')", apath, bpath,
-                    (int)(first + k + 1), name);
-    }
-    return 0;
-  };
+  RestartPair R;
+  int rc0 = R.open_(apath, bpath, P);
+  if (rc0) return rc0;
+  if (nstep) *nstep = R.nstep;
+  if (dtime) *dtime = R.dtime;
+  const bool old_format = R.old_format;
+  const int64_t NI = R.NI, NIJ = R.NIJ;
+  auto field = [&](const char* name, int n, double* dst) -> int { return R.field(name, n, dst); };
   int rc;
   if ((rc = field("u", 2 * kdm, F->u))) return rc;
   if ((rc = field("v", 2 * kdm, F->v))) return rc;
@@ -274,6 +295,16 @@ int hycom_restart_read(const char* apath, const char* bpath, const hyb_params* P
   if (ntr > 0 && F->tracer)
     if ((rc = field("tracer", 2 * kdm * ntr, F->tracer))) return rc;
   return 0;
+}
+
+int hycom_restart_read_field(const char* apath, const char* bpath, const hyb_params* P,
+                             const char* name, int32_t nrec, double* out) {
+  if (!apath || !bpath || !P || !name || !out) return fail(HYB_EINVAL, "null argument");
+  if (nrec < 1) return fail(HYB_EINVAL, "nrec=%d", nrec);
+  RestartPair R;
+  int rc = R.open_(apath, bpath, P);
+  if (rc) return rc;
+  return R.field(name, nrec, out);
 }
 
 }  // extern "C"

@@ -108,6 +108,11 @@ class HybTrcFields(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("si", "pi", "po", "mask", "so")]
 
 
+class HybStartFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("pbot", "dp", "ip", "iu", "iv", "depthu", "depthv", "p", "dpu", "dpv")]
+
+
 # every symbol include/hycom_hybgen.h declares, with its ctypes signature
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -161,6 +166,11 @@ SIGNATURES = {
                              + [ctypes.c_void_p] * 5),
     "hycom_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P(HybParams),
                                           _P(HybFields), ctypes.c_void_p, ctypes.c_void_p]),
+    "hycom_restart_read_field": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P(HybParams),
+                                                ctypes.c_char_p, ctypes.c_int32,
+                                                ctypes.c_void_p]),
+    "hycom_grid_masks": (ctypes.c_int, [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 5),
+    "hycom_state_start": (ctypes.c_int, [_P(HybParams), _P(HybStartFields)]),
     "hycom_trc_remap_device": (ctypes.c_int, [_P(HybTrcParams), _P(HybTrcFields),
                                               ctypes.c_void_p]),
 }

@@ -454,6 +454,58 @@ int hycom_za_read_device(const char* apath, int32_t idm, int32_t jdm, int32_t nb
 int hycom_restart_read(const char* apath, const char* bpath, const hyb_params* P,
                        const hyb_fields* F, int32_t* nstep, double* dtime);
 
+/* The nrec consecutive records of one named field of a restart pair (the
+ * name as restart_out writes it in columns 1-8 of the .b line, e.g. "pbot",
+ * "pbavg"), range-checked as above, into out(NI,NJ,nrec); halo points are
+ * not written.  restart_in3d (mod_restart.F90:581-617). */
+int hycom_restart_read_field(const char* apath, const char* bpath, const hyb_params* P,
+                             const char* name, int32_t nrec, double* out);
+
+/* ---------------------------------------------------------------------
+ * Model start on one tile holding the whole domain (i0 = j0 = 0, ii = idm,
+ * jj = jdm), host arrays (NI,NJ) with the halo:
+ *   1. hycom_grid_masks: the bathymetry as geopar.F90:229-256 reads it
+ *      (values above 2^99 are land) and bigrid.F90:1-330 turns it into the
+ *      p, u and v masks;
+ *   2. hycom_restart_read (th3d uses ip) and hycom_restart_read_field for
+ *      "pbot";
+ *   3. hycom_state_start: the fields mod_hycom.F90:1684-1722 derives before
+ *      the first step (depthu, depthv, the dp halo, p, dpu, dpv).
+ * The result is the state the model's first hybgen call sees.
+ * ------------------------------------------------------------------- */
+
+/* bigrid on the bathymetry depths (m): interior voids become 0
+ * (geopar.F90:244-252), the halo is filled as bigrid fills it (land beyond
+ * closed boundaries); ip, iu, iv are written at every point, halo included.
+ * *nreg gets the region type bigrid finds in the bathymetry: 1 if the last
+ * column has sea (periodic in i), 2 if the last row has sea too (global,
+ * arctic fold: the tripole halo of the model's ARCTIC build), else 0.
+ * HYB_EINVAL where bigrid stops: a sea point with 4 land neighbours, or
+ * sea in the last row of a domain that is not periodic.  The f-plane
+ * (mapflg 4) is not handled. */
+int hycom_grid_masks(int32_t idm, int32_t jdm, int32_t nbdy, double* depths, int32_t* ip,
+                     int32_t* iu, int32_t* iv, int32_t* nreg);
+
+typedef struct hyb_start_fields {
+  double *pbot;                  /* (NI,NJ) Pa, interior in; halo filled     */
+  double *dp;                    /* (NI,NJ,kdm,2) interior in; halo filled   */
+  const int32_t *ip, *iu, *iv;   /* (NI,NJ) from hycom_grid_masks            */
+  double *depthu, *depthv;       /* (NI,NJ) out, Pa                          */
+  double *p;                     /* (NI,NJ,kdm+1) out: level 2 at sea points,
+                                    p(:,:,1) = 0 everywhere; other land values
+                                    are left as they are                     */
+  double *dpu, *dpv;             /* (NI,NJ,kdm,2) out                        */
+} hyb_start_fields;
+
+/* mod_hycom.F90:1684-1722 with no open boundaries (lbflag 0): dpthuv's
+ * depthu = min(pbot, pbot(i-1)) and depthv at u and v points (dpthuv.F90:
+ * 48-123), the dp halo, then for time levels 1 and 2 p from dp at sea
+ * points and dpudpv (dpudpv.F90:85-100) out to nbdy-1.  Halos are those of
+ * xctilr with P->vland at closed boundaries.  P gives idm = ii, jdm = jj,
+ * nbdy, kdm, nreg (0, 1 or 2, from hycom_grid_masks) and vland (1.0 after
+ * geopar). */
+int hycom_state_start(const hyb_params* P, const hyb_start_fields* F);
+
 /* Same on device pointers, ordered on `stream`; returns after enqueueing
  * (a column that would stop is left NaN, not reported).  Calls on one
  * device share a workspace: do not overlap them on different streams. */

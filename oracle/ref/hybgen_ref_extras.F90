@@ -176,4 +176,108 @@
       href_za = 0
       end function href_za
 
+! Every array gindex_allocate (mod_dimensions.F90:118-210) allocates,
+! released first so that it can allocate them again at the current
+! extents (ensure_alloc allocates ip, iu, iv on its own).
+      subroutine gindex_reset
+      if (allocated(ip)) deallocate(ip)
+      if (allocated(iu)) deallocate(iu)
+      if (allocated(iv)) deallocate(iv)
+      if (allocated(iq)) deallocate(iq, iuopn, ivopn, ishlf, ipa, &
+          ipim1x, ipip1x, ipjm1x, ipjp1x, ipim1, ipip1, ipjm1, ipjp1, &
+          ifp, ilp, ifq, ilq, ifu, ilu, ifv, ilv, &
+          jfp, jlp, jfq, jlq, jfu, jlu, jfv, jlv, &
+          isp, isq, isu, isv, jsp, jsq, jsu, jsv, &
+          allip, alliq, alliu, alliv)
+      call gindex_allocate
+      end subroutine gindex_reset
+
+! The model start on one tile through the reference's own routines: the
+! bathymetry as geopar.F90:244-256 treats it (voids to land, then
+! bigrid), geopar's zero start of depthu, depthv, dpu, dpv and p(:,:,1)
+! (760-800), then mod_hycom.F90:1684-1722: dpthuv, the dp halo, and p and
+! dpudpv of both time levels.  All arrays are (NI,NJ[,...]) with the
+! driver's nbdy; depths, pbot and dp come back with their halos filled.
+      integer(c_int) function href_start(prm, depths_, pbot_, dp_, ip_, iu_, iv_, nreg_, &
+          depthu_, depthv_, p_, dpu_, dpv_) bind(C, name='href_start')
+      use hybgen_ref_driver, only: hyb_params_t, ensure_alloc, set_params, cin, cout
+      type(hyb_params_t), intent(in) :: prm
+      type(c_ptr), value :: depths_, pbot_, dp_, ip_, iu_, iv_
+      type(c_ptr), value :: depthu_, depthv_, p_, dpu_, dpv_
+      integer(c_int32_t), intent(out) :: nreg_
+      real, allocatable :: util1(:,:), util2(:,:), util3(:,:)
+      integer(c_int32_t), pointer :: io(:)
+      integer :: i, j, k, nm, n2, margin
+      href_start = -1
+      if (prm%nbdy .ne. nbdy) return
+      call ensure_alloc(prm)
+      call set_params(prm)
+      call gindex_reset
+      if (allocated(pbot)) deallocate(pbot, pbotmin, corio, pvtrop)
+      allocate(pbot(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy), &
+               pbotmin(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy), &
+               corio(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy), &
+               pvtrop(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy))
+      allocate(util1(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy), &
+               util2(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy), &
+               util3(1-nbdy:idm+nbdy,1-nbdy:jdm+nbdy))
+      n2 = (idm+2*nbdy)*(jdm+2*nbdy)
+      util1 = 0.0
+      util2 = 0.0
+      util3 = 0.0
+      call cin(depths, depths_, n2)
+      do j= 1,jj
+        do i= 1,ii
+          if (depths(i,j).gt.0.5*hugel) depths(i,j) = 0.0
+        enddo
+      enddo
+      nreg  = -1   ! TYPE=one: bigrid decides
+      vland = 0.0  ! xcspmd's value until geopar sets it after bigrid
+      call bigrid(depths, 0, util1, util2, util3)
+      nreg_ = nreg
+      vland = prm%vland
+      depthu = 0.0
+      depthv = 0.0
+      dpu = 0.0
+      dpv = 0.0
+      p(:,:,1) = 0.0
+      corio = 0.0
+      oneta0 = 0.0
+      lbflag = 0
+      call cin(pbot, pbot_, n2)
+      call dpthuv
+      call cin(dp, dp_, n2*kdm*2)
+      call xctilr(dp(1-nbdy,1-nbdy,1,1), 1, 2*kk, nbdy, nbdy, halo_ps)
+      margin = nbdy
+      do nm= 1,2
+        do j= 1-margin,jj+margin
+          do i= 1-margin,ii+margin
+            if (ip(i,j).ne.0) then
+              p(i,j,1) = 0.0
+              do k= 1,kk
+                p(i,j,k+1) = p(i,j,k)+dp(i,j,k,nm)
+              enddo
+            endif
+          enddo
+        enddo
+        call dpudpv(dpu(1-nbdy,1-nbdy,1,nm), dpv(1-nbdy,1-nbdy,1,nm), &
+                    p, depthu, depthv, margin, max(0,margin-1))
+      enddo
+      call cout(depths, depths_, n2)
+      call cout(pbot, pbot_, n2)
+      call cout(dp, dp_, n2*kdm*2)
+      call cout(depthu, depthu_, n2)
+      call cout(depthv, depthv_, n2)
+      call cout(p, p_, n2*(kdm+1))
+      call cout(dpu, dpu_, n2*kdm*2)
+      call cout(dpv, dpv_, n2*kdm*2)
+      call c_f_pointer(ip_, io, [n2])
+      io = reshape(ip, [n2])
+      call c_f_pointer(iu_, io, [n2])
+      io = reshape(iu, [n2])
+      call c_f_pointer(iv_, io, [n2])
+      io = reshape(iv, [n2])
+      href_start = 0
+      end function href_start
+
       end module hybgen_ref_extras

@@ -55,7 +55,9 @@ def ref_path(denfix=False):
 
 def ref(denfix=False):
     if denfix not in _refs:
-        lib = ctypes.CDLL(ref_path(denfix))
+        # lazy binding: dpthuv's calls of latbdt/latbdtf (lbflag 2 and 4,
+        # never taken here) stay unresolved
+        lib = ctypes.CDLL(ref_path(denfix), mode=os.RTLD_LAZY)
         lib.href_hybgen.restype = ctypes.c_int
         lib.href_hybgen.argtypes = [_P(abi.HybParams), _P(abi.HybFields), ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, _P(ctypes.c_double)]
@@ -118,6 +120,25 @@ def run_oracle_stokes(sd):
     if rc != 0:
         raise RuntimeError(f"oracle_stokes_vertical rc={rc}")
     return sd
+
+
+def start_ref(arctic=False):
+    """href_start of the reference build (the ARCTIC build of mod_xc for a
+    global grid), opened with lazy binding as ref() is."""
+    key = "start-arctic" if arctic else "start"
+    if key not in _refs:
+        path = os.path.join(REF_DIR, "libhybgen_ref_arctic.so") if arctic else ref_path(False)
+        lib = ctypes.CDLL(path, mode=os.RTLD_LAZY) if arctic else ref(False)
+        lib.href_start.restype = ctypes.c_int
+        lib.href_start.argtypes = [_P(abi.HybParams)] + [ctypes.c_void_p] * 6 + [
+            _P(ctypes.c_int32)] + [ctypes.c_void_p] * 5
+        _refs[key] = lib
+    return _refs[key]
+
+
+def start_ref_available(arctic=False):
+    return os.path.exists(os.path.join(REF_DIR, "libhybgen_ref_arctic.so" if arctic
+                                       else "libhybgen_ref.so"))
 
 
 TRC_REF_SO = os.path.join(REF_DIR, "libtrcupd_ref.so")
