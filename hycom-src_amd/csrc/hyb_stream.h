@@ -1595,7 +1595,7 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;  // pres(k-2), pres(k-1), pres(k)
   bool have = false;
   double cand = 0.0;
-#pragma unroll 2
+#pragma unroll 8
   for (int k = 1; k <= K; k++) {
     const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
     const double d = dmax(0.0, a - b);
@@ -1622,7 +1622,7 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
     UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
     walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
     double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
-#pragma unroll 2
+#pragma unroll 8
     for (int k = 1; k <= K; k++) {
       const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
       dpu[(int64_t)(k - 1) * S] = dmax(0.0, a - bb);

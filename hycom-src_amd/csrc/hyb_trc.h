@@ -175,32 +175,52 @@ HYB_HDI bool trc_remap_fields(const KTrc& Q, int64_t c0, int f0) {
       const double dz = zb - zt;
 #pragma unroll
       for (int f = 0; f < NF; f++) sok[f] = 0.0;
-      double pl1 = PI(lf);  // pi(l), carried from the iteration before
+      // pi(l), pi(l+1) and layer l's words are in registers when
+      // iteration l starts: each iteration loads layer l+1's while it sums
+      // layer l (a break leaves those loads unused)
+      double pl1 = PI(lf), pl2 = PI(lf + 1);
+      double cv[NF], cs[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        cv[f] = si[f][(int64_t)(lf - 1) * S];
+        cs[f] = (METHOD == 1) ? sis[S * ((int64_t)(lf - 1) * kTrcChunk + f)] : 0.0;
+      }
       for (int l = lf; l <= ki; l++) {
         const double pl = pl1;
         if (pl > zb) {
           lf = l - 1;
           break;
         }
-        pl1 = PI(l + 1);
+        pl1 = pl2;
+        const bool more = l + 1 <= ki;
+        double nv[NF], ns[NF];
+        pl2 = more ? PI(l + 2) : 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          nv[f] = more ? si[f][(int64_t)l * S] : 0.0;
+          ns[f] = (METHOD == 1 && more) ? sis[S * ((int64_t)l * kTrcChunk + f)] : 0.0;
+        }
         if (pl >= zt && pl1 <= zb) {
           const double q = dmax(pl1 - pl, thin) / dz;
 #pragma unroll
-          for (int f = 0; f < NF; f++) sok[f] = sok[f] + q * si[f][(int64_t)(l - 1) * S];
+          for (int f = 0; f < NF; f++) sok[f] = sok[f] + q * cv[f];
         } else {
           const double bot = dmin(pl1, zb), top = dmax(pl, zt);
           const double q = dmax(bot - top, thin) / dz;
           if (METHOD == 0) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) sok[f] = sok[f] + q * si[f][(int64_t)(l - 1) * S];
+            for (int f = 0; f < NF; f++) sok[f] = sok[f] + q * cv[f];
           } else {
             const double zc = 0.5 * (bot + top);
             const double qc = (zc - pl) / dmax(pl1 - pl, thin) - 0.5;
 #pragma unroll
-            for (int f = 0; f < NF; f++)
-              sok[f] = sok[f] + q * (si[f][(int64_t)(l - 1) * S] +
-                                     qc * sis[S * ((int64_t)(l - 1) * kTrcChunk + f)]);
+            for (int f = 0; f < NF; f++) sok[f] = sok[f] + q * (cv[f] + qc * cs[f]);
           }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          cv[f] = nv[f];
+          cs[f] = ns[f];
         }
       }
 #pragma unroll
