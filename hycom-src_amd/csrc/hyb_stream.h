@@ -724,6 +724,7 @@ struct PoRecompute {
   double depthu;
   int k1;                                // kk+1, the deepest level
   double prv = 0.0, cur = 0.0, b = 0.0;  // prsf(k-1), prsf(k), min(depthu, p-mean at k)
+  double dl = 0.0;                       // the new thickness of layer k-1 (cur = prv + dl)
   double q0 = 0.0, q1 = 0.0;             // p at level k+1 of both points, loaded ahead
   int k = 0;                             // 0 until the first request
   HYB_HD PoRecompute(const double* a0, const double* a1, int64_t s, double du, int kk)
@@ -732,6 +733,7 @@ struct PoRecompute {
     if (kq == 1 || k == 0) {
       prv = 0.0;
       cur = 0.0;
+      dl = 0.0;
       b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
       k = 1;
       q0 = p0[S];
@@ -743,6 +745,7 @@ struct PoRecompute {
       b = a;
       prv = cur;
       cur = cur + d;
+      dl = d;
       k++;
       if (k < k1) {
         q0 = p0[(int64_t)k * S];
@@ -817,6 +820,13 @@ struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
   double ptop;  // prsf(kk+1)
   const PO* po;
   uint64_t fix = 0;
+  // With PoRecompute, the new dpu of layer kc (dpudpv.F90:92-103) is
+  // written as output kc is: the walk asks for prsf(kc+1) just before it
+  // emits output kc, so the thickness that formed it is po->dl.  The walk
+  // has read the old dpu of layer kc by then: outputs end no higher than
+  // two layers above their index (the in-place test of uv_stream), and the
+  // old interfaces are summed PD+3 layers ahead of the walk.
+  double* dpn = nullptr;
   HYB_HD void emit(int kc, const double* v, bool certain, bool thick) {
     if (thick || po->prv <= ptop - kOnemm) {
       uu[(int64_t)(kc - 1) * S] = v[0];
@@ -824,6 +834,7 @@ struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
     } else {
       uu[(int64_t)(kc - 1) * S] = -ub;
     }
+    if constexpr (std::is_same<PO, PoRecompute>::value) dpn[(int64_t)(kc - 1) * S] = po->dl;
   }
   HYB_HD void fixup(const double* lo, const double* hi, const double* smin,
                     const double* smax) {
@@ -1619,15 +1630,8 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
   double* fp[1] = {uu};
   if (RECOMP) {
     PoRecompute po(p0, p1, S, depthu, K);
-    UvSink<PoRecompute> sink{uu, S, ub, pr, &po};
+    UvSink<PoRecompute> sink{uu, S, ub, pr, &po, 0ull, dpu};
     walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
-    double bb = dmin(depthu, 0.5 * (p0[0] + p1[0]));
-#pragma unroll 8
-    for (int k = 1; k <= K; k++) {
-      const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
-      dpu[(int64_t)(k - 1) * S] = dmax(0.0, a - bb);
-      bb = a;
-    }
   } else {
     PoSum po{dcol};
     UvSink<PoSum> sink{uu, S, ub, pr, &po};
