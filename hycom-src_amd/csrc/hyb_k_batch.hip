@@ -109,11 +109,10 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   const int64_t c0 = ij_of(V, i, r.j);
-  if (V.ip[c0] == 0) return;
-  if (W.kp[c0] & kUnsafe) return;
+  const int32_t ipw = V.ip[c0], kpw = W.kp[c0];  // kp of a land column: read, not used
+  if ((ipw == 0) | ((kpw & kUnsafe) != 0)) return;
   double* row = lds + threadIdx.x;
-  const double* pg = V.p + c0;
-  for (int k = 1; k <= P.kk + 1; k++) row[(k - 1) * CW] = pg[(int64_t)(k - 1) * V.NIJ];
+  stage_column(row, CW, V.p + c0, V.NIJ, P.kk + 1);
   LaneRow po{row, CW};
   if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
   else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);

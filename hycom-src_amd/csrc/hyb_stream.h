@@ -1553,6 +1553,21 @@ HYB_HDI void remap_fields(const KParams& P, const View& V, const Scratch& W, int
                                            po);
 }
 
+// Copies n levels of a column (level stride S) into a lane's LDS column
+// (stride ld), 16 loads in flight before the first is stored: a column of
+// up to 65 levels waits on memory at most five times instead of once per
+// level or four.
+HYB_HDI void stage_column(double* row, int ld, const double* pg, int64_t S, int n) {
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    double t[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) t[q] = (k0 + q < n) ? pg[(int64_t)(k0 + q) * S] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (k0 + q < n) row[(int64_t)(k0 + q) * ld] = t[q];
+  }
+}
+
 // ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
 // walk of walk_core with FAST set.  The reconstruction (coef_layer) runs
