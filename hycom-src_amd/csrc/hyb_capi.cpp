@@ -570,10 +570,12 @@ int plan_build(BatchPlan& b, const hyb_domain* D, int ndom, std::vector<unsigned
   b.B = BatchDevice{b.dP, b.dV, b.dW};
   int kmax = 0;
   for (int d = 0; d < ndom; d++) kmax = b.hP[d].kk > kmax ? b.hP[d].kk : kmax;
-  // candidate widths; a remap workgroup holds (kmax+1) x cols doubles of LDS
+  // candidate widths; a remap workgroup holds levels 2..kmax of its columns'
+  // interfaces in LDS, (kmax-1) x cols doubles
   b.cands.clear();
+  const int lvl = kmax > 1 ? kmax - 1 : 1;
   for (int rc : {64, 128, 256})
-    if (sizeof(double) * (kmax + 1) * rc <= 65536)
+    if (sizeof(double) * lvl * rc <= 65536)
       for (int gc : {256, 128, 64}) b.cands.push_back({rc, gc});
   b.ms.assign(b.cands.size(), -1.0f);
   b.cand = 0;

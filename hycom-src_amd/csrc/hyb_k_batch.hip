@@ -111,9 +111,7 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   const int64_t c0 = ij_of(V, i, r.j);
   const int32_t ipw = V.ip[c0], kpw = W.kp[c0];  // kp of a land column: read, not used
   if ((ipw == 0) | ((kpw & kUnsafe) != 0)) return;
-  double* row = lds + threadIdx.x;
-  stage_column(row, CW, V.p + c0, V.NIJ, P.kk + 1);
-  LaneRow po{row, CW};
+  StagedInterfaces po = stage_interfaces(lds + threadIdx.x, CW, V.p + c0, V.NIJ, P.kk);
   if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
   else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
 }
@@ -216,7 +214,7 @@ static void remap_group_cw(const BatchDevice& B, const BatchGroup& G, int n, siz
 
 static void remap_group(const BatchDevice& B, const BatchGroup& G, int n, int kmax, int cw,
                         hipStream_t s) {
-  const size_t lds = sizeof(double) * (kmax + 1) * cw;
+  const size_t lds = sizeof(double) * lds_levels(kmax) * cw;
   if (cw == 256) remap_group_cw<256>(B, G, n, lds, s);
   else if (cw == 128) remap_group_cw<128>(B, G, n, lds, s);
   else remap_group_cw<64>(B, G, n, lds, s);

@@ -1568,6 +1568,28 @@ HYB_HDI void stage_column(double* row, int ld, const double* pg, int64_t S, int 
   }
 }
 
+// The new interfaces of a column as the walk reads them: levels 2..K
+// staged in the lane's LDS column, levels 1 and K+1 kept in registers.
+// K-1 doubles per lane: at K = 41 a 64-lane wave takes 20 KB of LDS, so a
+// CU's 160 KB holds the 8 waves its registers allow (K+1 levels held 7).
+HYB_HD int lds_levels(int K) { return K > 1 ? K - 1 : 1; }
+struct StagedInterfaces {
+  const double* base;  // level 2 of the lane's column
+  int stride, k1;      // LDS stride; K+1
+  double top, bot;     // levels 1 and K+1
+  HYB_HD double operator()(int k) const {
+    if (k == 1) return top;
+    if (k >= k1) return bot;
+    return base[(int64_t)(k - 2) * stride];
+  }
+};
+HYB_HDI StagedInterfaces stage_interfaces(double* row, int ld, const double* pg, int64_t S,
+                                          int K) {
+  const double top = pg[0], bot = pg[(int64_t)K * S];
+  stage_column(row, ld, pg + S, S, K - 1);
+  return StagedInterfaces{row, ld, K + 1, top, bot};
+}
+
 // ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
 // walk of walk_core with FAST set.  The reconstruction (coef_layer) runs
