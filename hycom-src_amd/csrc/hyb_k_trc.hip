@@ -7,7 +7,7 @@
 
 namespace hyb {
 
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) k_trc(KTrc Q) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) k_trc(KTrc Q) {
   const int j = blockIdx.y + 1;
   const int i0 = 1 - (int)(((int64_t)Q.nb + Q.NI * (int64_t)(j - 1 + Q.nb)) & 15);
   const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
