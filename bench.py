@@ -475,7 +475,7 @@ def main():
                     help="streaming remap variant 0..16 (hycom_hybgen_tune knob 0; 3 is "
                          "the library default, 16 the tiled remap)")
     ap.add_argument("--uv-prefetch", type=int, default=None,
-                    help="streaming u/v prefetch depth 0 or 2 (knob 1)")
+                    help="streaming u/v prefetch depth 0, 2 or 4 (knob 1)")
     ap.add_argument("--regrid-ahead", type=int, default=None,
                     help="regrid window loaded one iteration ahead (1) or not (0; knob 3)")
     ap.add_argument("--concurrent-uv", type=int, default=None,

@@ -735,7 +735,8 @@ int hycom_hybgen_tune(int knob, int value) {
     return 0;
   }
   if (knob == 1) {
-    if (value != 0 && value != 2) return fail(HYB_EINVAL, "u/v prefetch %d: 0 or 2", value);
+    if (value != 0 && value != 2 && value != 4)
+      return fail(HYB_EINVAL, "u/v prefetch %d: 0, 2 or 4", value);
     g_tune.uv_prefetch = value;
     return 0;
   }

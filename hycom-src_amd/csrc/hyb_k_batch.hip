@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kWave) kb_uv(BatchDevice B, BatchGroup G, int 
   const int32_t* msk = isv ? V.iv : V.iu;
   if (msk[c0] == 0) return;
   LaneRow dcol{lds + threadIdx.x, kWave};  // unused: new thicknesses formed again from p
-  const bool ok = uv_stream<METHOD, 2, MIXED, true>(P, V, i, j, nl, isv, dcol);
+  const bool ok = uv_stream<METHOD, 4, MIXED, true>(P, V, i, j, nl, isv, dcol);
   gscratch(B.W, d).uvf[c0 + (isv ? V.NIJ : 0)] = ok ? 0 : 1;
 }
 

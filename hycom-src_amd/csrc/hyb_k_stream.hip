@@ -77,7 +77,9 @@ static void uv_stream_m(const KParams& P, const View& V, const Scratch& W, int n
   dim3 gw3((P.ii + kAlignPad + kWave - 1) / kWave, P.jj, 2);
   const size_t lds = RECOMP ? 0 : sizeof(double) * P.kk * kWave;
   if (P.precision == 1)
-    hipLaunchKernelGGL((k_uv_s<M, 2, true, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
+    hipLaunchKernelGGL((k_uv_s<M, 4, true, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
+  else if (pd == 4)
+    hipLaunchKernelGGL((k_uv_s<M, 4, false, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
   else if (pd > 0)
     hipLaunchKernelGGL((k_uv_s<M, 2, false, RECOMP>), gw3, w1, lds, s, P, V, W, nl);
   else

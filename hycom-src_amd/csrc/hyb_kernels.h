@@ -36,10 +36,10 @@ bool stream_path_ok(const KParams& P);
 int uv_method_of(const KParams& P);
 // Kernel variants of the streaming family (measurement and tuning):
 // remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
-// uv_prefetch the prefetch depth of the u/v walk (0 or 2).
+// uv_prefetch the prefetch depth of the u/v walk (0, 2 or 4).
 struct StreamTuning {
   int remap_variant = 3;  // 3 fields per walk, 2 levels loaded ahead (fastest measured)
-  int uv_prefetch = 2;
+  int uv_prefetch = 4;
   int uv_recompute = 1;  // u/v new thicknesses formed again from p, no LDS (fastest measured)
   int regrid_ahead = 1;  // the regrid's window loaded one iteration ahead (fastest measured)
   int concurrent_uv = 0; // the u/v phase on a second stream beside the p-column remap

@@ -193,7 +193,7 @@ int hycom_hybgen_set_kernel_path(int path);
  * compiler (6..13), 14 = (3,4) and 15 = (3,2) with non-temporal output
  * stores, both staged in LDS, 16 = the tiled layer-parallel remap (PPM in
  * FP64 with 4 <= kdm <= 63; otherwise 3); knob 1: levels loaded ahead by the
- * u/v walk, 0 or 2; knob 2: the u/v walk's new thicknesses kept in LDS (0)
+ * u/v walk, 0, 2 or 4 (the default); knob 2: the u/v walk's new thicknesses kept in LDS (0)
  * or formed again from p as the walk needs them (1, no LDS); knob 3: the
  * regrid's window loaded one iteration ahead (1) or not (0); knob 4: the
  * u/v phase on a second stream of the library's, beside the p-column remap

@@ -208,6 +208,26 @@ def test_streaming_uv_with_lds(gpu, kdm, hybmap, extreme):
     _assert_same(want, got, f"uv recompute kdm={kdm} hybmap={hybmap}")
 
 
+def test_uv_prefetch_depths(gpu):
+    """The u/v walk at prefetch depths 0, 2 and 4 (knob 1; 4 is the
+    default): bitwise the oracle."""
+    P, w = _case("cfg1-uv-prefetch", hybmap=3)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=106))
+    want = _oracle_out(d0)
+    lib = abi.load()
+    import torch
+    try:
+        for pd in (0, 2, 4):
+            print(f"[uv prefetch] {pd}", flush=True)
+            abi.check(lib.hycom_hybgen_tune(1, pd))
+            dd = abi.DeviceDomain(P).upload(d0)
+            dd.hybgen(m=1, n=2)
+            torch.cuda.synchronize()
+            _assert_same(want, dd.download(), f"uv prefetch {pd}")
+    finally:
+        abi.check(lib.hycom_hybgen_tune(1, 4))
+
+
 @pytest.mark.parametrize("hybmap,raflag", [(2, 0), (3, 1), (1, 0), (0, 1)])
 def test_concurrent_uv_phase(gpu, hybmap, raflag):
     """The u/v phase on the library's second stream beside the p-column
