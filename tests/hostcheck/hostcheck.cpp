@@ -293,15 +293,15 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
           bool ok;
           if (P.precision == 1) {
             switch (uv_method(P)) {
-              case 0: ok = uv_stream<0, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
-              case 1: ok = uv_stream<1, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
-              default: ok = uv_stream<3, 2, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 0: ok = uv_stream<0, 4, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 1: ok = uv_stream<1, 4, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              default: ok = uv_stream<3, 4, true, true>(P, V, i, j, nl, isv != 0, dcol); break;
             }
           } else if (g_uv_recompute) {
             switch (uv_method(P)) {
-              case 0: ok = uv_stream<0, 2, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              case 0: ok = uv_stream<0, 4, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
               case 1: ok = uv_stream<1, 0, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
-              default: ok = uv_stream<3, 2, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
+              default: ok = uv_stream<3, 4, false, true>(P, V, i, j, nl, isv != 0, dcol); break;
             }
           } else switch (uv_method(P)) {
             case 0: ok = uv_stream<0, 2>(P, V, i, j, nl, isv != 0, dcol); break;
