@@ -27,12 +27,7 @@ METRIC = "column-layers remapped/s, FP64 vs mixed, GLBb tile; p50/p99 request la
 HBM_PEAK_GBS = 8000.0
 PHASES = ["column", "halo", "uv", "ra", "halo2", "uv2", "regrid", "remap"]
 # kernel of each p-column remap variant (hycom_hybgen_tune knob 0)
-REMAP_KERNELS = {0: "k_remap_s<{m},6,0>", 1: "k_remap_s<{m},6,2>", 2: "k_remap_s<{m},3,0>",
-                 3: "k_remap_s<{m},3,2>", 4: "k_remap_s<{m},2,2>", 5: "k_remap_s<{m},2,4>",
-                 6: "k_remap_g<{m},3,0,1>", 7: "k_remap_g<{m},3,0,3>", 8: "k_remap_g<{m},2,0,3>",
-                 9: "k_remap_g<{m},2,2,3>", 10: "k_remap_g<{m},2,0,4>", 11: "k_remap_g<{m},3,2,1>",
-                 12: "k_remap_g<{m},3,2,3>", 13: "k_remap_g<{m},2,2,4>",
-                 14: "k_remap_s<{m},3,4>", 15: "k_remap_n<{m},3,2>", 16: "k_remap_t<3>"}
+REMAP_KERNELS = {3: "k_remap_s<{m},3,2>"}
 
 
 def bytes_per_column_phase_a(K, T):
@@ -470,10 +465,10 @@ def main():
     ap.add_argument("--serve-domains", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--kernel-path", type=int, default=0,
-                    help="0 auto, 1 lane arrays, 2 register-resident, 3 streaming")
+                    help="0 auto, 1 lane arrays, 3 streaming")
     ap.add_argument("--remap-variant", type=int, default=3,
-                    help="streaming remap variant 0..16 (hycom_hybgen_tune knob 0; 3 is "
-                         "the library default, 16 the tiled remap)")
+                    help="streaming remap variant (hycom_hybgen_tune knob 0; 3, the "
+                         "library default, is the only one built)")
     ap.add_argument("--uv-prefetch", type=int, default=None,
                     help="streaming u/v prefetch depth 0, 2 or 4 (knob 1)")
     ap.add_argument("--regrid-ahead", type=int, default=None,

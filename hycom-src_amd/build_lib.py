@@ -32,8 +32,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
           "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["hyb_k_batch.hip", "hyb_k_remap_m2.hip", "hyb_k_remap_m3.hip", "hyb_k_remap_m1.hip",
-           "hyb_k_remap_m0.hip", "hyb_k_stream.hip", "hyb_k_lane.hip", "hyb_k_fast.hip",
-           "hyb_k_misc.hip", "hyb_k_tile.hip", "hyb_k_stokes.hip", "hyb_k_trc.hip", "hyb_k_za.hip",
+           "hyb_k_remap_m0.hip", "hyb_k_stream.hip", "hyb_k_lane.hip",
+           "hyb_k_misc.hip", "hyb_k_stokes.hip", "hyb_k_trc.hip", "hyb_k_za.hip",
            "hyb_capi.cpp", "hyb_server.cpp", "hyb_stokes.cpp", "hyb_trc.cpp", "hyb_za.cpp", "hyb_start.cpp"]
 
 

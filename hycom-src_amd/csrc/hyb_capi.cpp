@@ -218,7 +218,7 @@ PhaseTimer* timer_take() {
   return t;
 }
 
-// Workspace of the register-resident kernels (pres, kp, lcm), one per
+// Workspace of the streaming kernels (pres, kp, lcm, uvf), one per
 // domain, keyed by the domain's p array; allocated on first use.
 struct Workspace {
   const void* key = nullptr;
@@ -231,7 +231,7 @@ struct Workspace {
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
-int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays, 2 register-resident, 3 streaming
+int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays, 3 streaming (2 was register-resident)
 StreamTuning g_tune;
 
 int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
@@ -700,22 +700,14 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   V.rec_fixlay = g_rec_fixlay;
   V.rec_kp = g_rec_kp;
   const int path = g_kernel_path;
-  if (path == 2 && !fast_path_ok(K))
-    return fail(HYB_EUNSUPPORTED, "register-resident kernels do not cover kdm=%d/ldenfix/nhybrd",
-                P->kdm);
   if (path == 3 && !stream_path_ok(K))
     return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
   const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
-  const bool fast = path == 2;
   if (use_stream) {
     Scratch W;
     if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
     HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, g_tune, (hipStream_t)stream,
                                 timer_take()));
-  } else if (fast) {
-    Scratch W;
-    if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
-    HIPCHK(launch_hybgen_fast(K, V, W, m, n, raflag, (hipStream_t)stream, timer_take()));
   } else {
     HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream, timer_take()));
   }
@@ -724,13 +716,14 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
 
 int hycom_hybgen_set_kernel_path(int path) {
   if (path < 0 || path > 3) return fail(HYB_EINVAL, "kernel path %d", path);
+  if (path == 2) return fail(HYB_EUNSUPPORTED, "kernel path 2 (register-resident) was removed");
   g_kernel_path = path;
   return 0;
 }
 
 int hycom_hybgen_tune(int knob, int value) {
   if (knob == 0) {
-    if (value < 0 || value > 16) return fail(HYB_EINVAL, "remap variant %d outside 0..16", value);
+    if (value != 3) return fail(HYB_EINVAL, "remap variant %d: only 3 is built", value);
     g_tune.remap_variant = value;
     return 0;
   }

@@ -1072,4 +1072,20 @@ HYB_HD void p_halo_icol(const KParams& P, const View& V, int j, int k) {
   }
 }
 
+// Scratch shared by the streaming column kernels.  pres level k of column
+// c0 is pres[c0 + (k-1)*NIJ]; the record words are indexed by c0.
+struct Scratch {
+  double* pres;       // (NIJ, K+1) old interfaces after overturn/unmix
+  int32_t* kp;        // deepest inflated layer after overturn
+  uint64_t* lcm;      // PCM-layer mask after the dx0k pass
+  int32_t* uvf;       // (2, NIJ) u/v points left to the copy-first kernel
+};
+
+// Row of per-lane storage in LDS (device) or a local array (host).
+struct LaneRow {
+  double* base;
+  int stride;
+  HYB_HD double& operator()(int k) const { return base[(int64_t)(k - 1) * stride]; }
+};
+
 }  // namespace hyb

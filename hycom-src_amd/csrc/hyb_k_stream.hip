@@ -142,11 +142,8 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
     }
     {
       PhaseMark pk(T, s, 7);
-      // variant 16: the tiled remap where it applies, else the default walk
-      const int rv = tune.remap_variant == 16 ? 3 : tune.remap_variant;
-      if (tune.remap_variant == 16 && tile_remap_ok(P))
-        launch_remap_tiled(P, V, W, n, s);
-      else switch (P.hybmap) {
+      const int rv = tune.remap_variant;
+      switch (P.hybmap) {
         case 0: launch_remap_stream_m0(P, V, W, n, rv, s); break;
         case 1: launch_remap_stream_m1(P, V, W, n, rv, s); break;
         case 2: launch_remap_stream_m2(P, V, W, n, rv, s); break;

@@ -4,7 +4,6 @@
 #include <hip/hip_runtime_api.h>
 
 #include "hyb_column.h"
-#include "hyb_fast.h"
 #include "hyb_synth.h"
 
 namespace hyb {
@@ -22,20 +21,13 @@ struct PhaseTimer {
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,
                               int raflag, hipStream_t s, PhaseTimer* T = nullptr);
 
-// True when the register-resident kernels cover this parameter pack.
-bool fast_path_ok(const KParams& P);
-// Enqueues hybgen with the register-resident kernels; W must hold
-// NIJ*(kk+1) doubles (pres), NIJ int32 (kp) and NIJ uint64 (lcm).
-hipError_t launch_hybgen_fast(const KParams& P, const View& V, const Scratch& W, int m,
-                              int n, int raflag, hipStream_t s, PhaseTimer* T = nullptr);
-
 // True when the streaming kernels cover this parameter pack (any kdm,
 // nhybrd==kdm, no HYBGEN_DENFIX).
 bool stream_path_ok(const KParams& P);
 // remapper of the u/v phase: 0 PCM, 1 PLM, 3 PPM (hybgen.F90:1960-1975)
 int uv_method_of(const KParams& P);
 // Kernel variants of the streaming family (measurement and tuning):
-// remap_variant picks fields per walk and prefetch depth (hyb_k_remap.inc),
+// remap_variant picks the p-column remap kernel (hyb_k_remap.inc),
 // uv_prefetch the prefetch depth of the u/v walk (0, 2 or 4).
 struct StreamTuning {
   int remap_variant = 3;  // 3 fields per walk, 2 levels loaded ahead (fastest measured)
@@ -44,8 +36,8 @@ struct StreamTuning {
   int regrid_ahead = 1;  // the regrid's window loaded one iteration ahead (fastest measured)
   int concurrent_uv = 0; // the u/v phase on a second stream beside the p-column remap
 };
-// Enqueues hybgen with the streaming kernels; W as for launch_hybgen_fast
-// plus uvf (2*NIJ int32).
+// Enqueues hybgen with the streaming kernels; W must hold NIJ*(kk+1)
+// doubles (pres), NIJ int32 (kp), NIJ uint64 (lcm) and 2*NIJ int32 (uvf).
 hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& W, int m,
                                 int n, int raflag, const StreamTuning& tune, hipStream_t s,
                                 PhaseTimer* T = nullptr);

@@ -50,9 +50,5 @@ void launch_remap_stream_m2(const KParams& P, const View& V, const Scratch& W, i
                             int variant, hipStream_t s);
 void launch_remap_stream_m3(const KParams& P, const View& V, const Scratch& W, int n,
                             int variant, hipStream_t s);
-// hyb_k_tile.hip: the tiled PPM remap (FP64, 4 <= kdm <= 63)
-bool tile_remap_ok(const KParams& P);
-void launch_remap_tiled(const KParams& P, const View& V, const Scratch& W, int n,
-                        hipStream_t s);
 
 }  // namespace hyb

@@ -34,7 +34,6 @@
 #include "hyb_column.h"
 #include <type_traits>
 
-#include "hyb_fast.h"
 
 namespace hyb {
 

@@ -74,111 +74,15 @@ extern "C" int hostcheck_hybgen(const hyb_params* P, const hyb_fields* F, int m,
   return 0;
 }
 
-// ---- register-resident kernels (hyb_fast.h), same phase order ----
-#include "hyb_fast.h"
-#include <vector>
-
-template <int K>
-static void run_fast(const KParams& P, const View& V, int m, int n, int raflag) {
-  const int ii = P.ii, jj = P.jj, kk = P.kk;
-  std::vector<double> pres((size_t)V.NIJ * (K + 1));
-  std::vector<int32_t> kpv((size_t)V.NIJ);
-  std::vector<uint64_t> lcmv((size_t)V.NIJ);
-  Scratch W{pres.data(), kpv.data(), lcmv.data()};
-  const int itr[3] = {P.itr801, P.itr802, P.itr803};
-  double* src[3] = {V.dp, V.temp, V.saln};
-  for (int d = 0; d < 3; d++)
-    if (itr[d] > 0)
-      for (int j = 1; j <= jj; j++)
-        for (int i = 1; i <= ii; i++)
-          for (int k = 1; k <= kk; k++)
-            V.tracer[ij_of(V, i, j) + levt(V, k, n, itr[d])] = src[d][ij_of(V, i, j) + lev4(V, k, n)];
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int j = 1; j <= jj; j++)
-    for (int i = 1; i <= ii; i++)
-      if (V.ip[ij_of(V, i, j)]) regrid_fast<K>(P, V, W, i, j, n);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int j = 1; j <= jj; j++) {
-    double row[K + 2];
-    LaneRow lr{row, 1};
-    for (int i = 1; i <= ii; i++) {
-      if (!V.ip[ij_of(V, i, j)]) continue;
-      switch (P.hybmap) {
-        case 0: remap_fast<K, 0>(P, V, W, i, j, n, lr); break;
-        case 1: remap_fast<K, 1>(P, V, W, i, j, n, lr); break;
-        case 2: remap_fast<K, 2>(P, V, W, i, j, n, lr); break;
-        default: remap_fast<K, 3>(P, V, W, i, j, n, lr); break;
-      }
-    }
-  }
-  for (int d = 0; d < 3; d++)
-    if (itr[d] > 0) {
-      double q = d == 0 ? P.diag_q801 : P.diag_q80x;
-      for (int j = 1; j <= jj; j++)
-        for (int i = 1; i <= ii; i++)
-          if (V.ip[ij_of(V, i, j)])
-            for (int k = 1; k <= kk; k++) {
-              int64_t a = ij_of(V, i, j) + levt(V, k, n, itr[d]);
-              V.tracer[a] = q * (src[d][ij_of(V, i, j) + lev4(V, k, n)] - V.tracer[a]);
-            }
-    }
-  for (int pass = 0; pass <= (raflag ? 1 : 0); pass++) {
-    int nl = pass == 0 ? n : m;
-    if (pass == 1) {
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int j = 1; j <= jj; j++)
-        for (int i = 1; i <= ii; i++)
-          if (V.ip[ij_of(V, i, j)]) column_ra<K>(P, V, i, j, m, n, row_lcm(P));
-    }
-    for (int k = 2; k <= kk + 1; k++)
-      for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
-    for (int k = 2; k <= kk + 1; k++)
-      for (int j = 0; j <= jj + 1; j++) p_halo_icol(P, V, j, k);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int j = 1; j <= jj; j++) {
-      double row[K + 2];
-      LaneRow lr{row, 1};
-      for (int isv = 0; isv <= 1; isv++)
-        for (int i = 1; i <= ii; i++)
-          if ((isv ? V.iv : V.iu)[ij_of(V, i, j)]) uv_fast<K>(P, V, i, j, nl, isv != 0, lr);
-    }
-  }
-}
-
-extern "C" int hostcheck_hybgen_fast(const hyb_params* P, const hyb_fields* F, int m, int n,
-                                     int raflag) {
-  KParams K = kparams_of(P);
-  View V = view_of(P, F);
-  if (K.ldenfix || K.nhybrd != K.kk) return -2;
-  switch (K.kk) {
-    case 22: run_fast<22>(K, V, m, n, raflag); return 0;
-    case 32: run_fast<32>(K, V, m, n, raflag); return 0;
-    case 41: run_fast<41>(K, V, m, n, raflag); return 0;
-    default: return -2;
-  }
-}
-
 // ---- streaming kernels (hyb_stream.h), same phase order ----
 #include "hyb_stream.h"
-#include "hyb_tile.h"
+#include <vector>
 
 #include <memory>
 
 static int64_t g_unsafe_cols = 0;
 static int g_uv_recompute = 0;  // 1: the read-ahead regrid, remap and u/v variants
-static int g_tiled = 0;         // 1: the tiled PPM remap where it applies
 
-// the tiled remap of row j (hyb_tile.h), its 256 threads run in turn
-template <int Q>
-static void tile_row(const KParams& P, const View& V, const Scratch& W, int j, int n) {
-  auto team = std::make_unique<TileTeamHost<TileThread<Q>>>();
-  std::vector<double> lds(TileLayout{P.kk}.bytes() / sizeof(double) + 1);
-  for (int i0 = i_aligned0(V, j); i0 <= P.ii; i0 += kTileW)
-    tile_remap_ppm<Q>(P, V, W, n, j, i0, lds.data(), *team);
-}
-static bool tile_applies(const KParams& P) {
-  return P.hybmap == 2 && P.precision == 0 && P.kk >= 4 && P.kk + 1 <= 4 * kTileR;
-}
 static int64_t g_uv_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
@@ -203,21 +107,6 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
         else regrid_stream(P, V, W, i, j, n);
       }
   int64_t unsafe = 0;
-  if (g_tiled && tile_applies(P)) {
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : unsafe)
-    for (int j = 1; j <= jj; j++) {
-      for (int i = 1; i <= ii; i++) {
-        const int64_t c0 = ij_of(V, i, j);
-        if (V.ip[c0] && (W.kp[c0] & kUnsafe)) {
-          unsafe++;
-          remap_fallback(P, V, W, i, j, n);
-        }
-      }
-      if (kk + 1 <= 2 * kTileR) tile_row<2>(P, V, W, j, n);
-      else if (kk + 1 <= 3 * kTileR) tile_row<3>(P, V, W, j, n);
-      else tile_row<4>(P, V, W, j, n);
-    }
-  } else
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : unsafe)
   for (int j = 1; j <= jj; j++) {
     double row[kKMax + 2];
@@ -337,17 +226,6 @@ extern "C" int hostcheck_hybgen_stream_rc(const hyb_params* P, const hyb_fields*
   return rc;
 }
 
-// the same with the tiled p-column remap
-extern "C" int hostcheck_hybgen_stream_tiled(const hyb_params* P, const hyb_fields* F, int m,
-                                             int n, int raflag) {
-  g_uv_recompute = 1;
-  g_tiled = 1;
-  const int rc = hostcheck_hybgen_stream(P, F, m, n, raflag);
-  g_tiled = 0;
-  g_uv_recompute = 0;
-  return rc;
-}
-
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }
 
 extern "C" int64_t hostcheck_stream_uv_fallback_points(void) { return g_uv_fallback; }
@@ -390,48 +268,6 @@ extern "C" void hostcheck_remap_column(int mixed, int method, int kk, const doub
   }
 }
 
-// ---- one column through the tiled PPM remap (column-level checks) ----
-// The column sits alone in a 16-wide row (nb = 0); T and S both hold si.
-extern "C" void hostcheck_tile_column(int kk, const double* si, const double* pi,
-                                      const double* po, const int* lcm0, double* so) {
-  const int64_t NIJ = kTileW;
-  std::vector<double> temp(NIJ * kk * 2, 0.0), saln(NIJ * kk * 2, 0.0), th(NIJ * kk * 2, 0.0);
-  std::vector<double> p(NIJ * (kk + 1), 0.0), pres(NIJ * (kk + 1), 0.0);
-  std::vector<int32_t> ip(NIJ, 0), kp(NIJ, 0);
-  std::vector<uint64_t> lcm(NIJ, 0);
-  ip[0] = 1;
-  kp[0] = kk;
-  for (int k = 0; k < kk; k++) {
-    temp[k * NIJ] = si[k];
-    saln[k * NIJ] = si[k];
-    if (lcm0 && lcm0[k]) lcm[0] |= 1ull << k;
-  }
-  for (int k = 0; k <= kk; k++) {
-    pres[k * NIJ] = pi[k];
-    p[k * NIJ] = po[k];
-  }
-  KParams P{};
-  P.ii = 1;
-  P.jj = 1;
-  P.kk = kk;
-  P.nhybrd = kk;
-  P.hybmap = 2;
-  View V{};
-  V.temp = temp.data();
-  V.saln = saln.data();
-  V.th3d = th.data();
-  V.p = p.data();
-  V.ip = ip.data();
-  V.NI = NIJ;
-  V.NIJ = NIJ;
-  V.nb = 0;
-  V.K = kk;
-  Scratch W{pres.data(), kp.data(), lcm.data(), nullptr};
-  if (kk + 1 <= 2 * kTileR) tile_row<2>(P, V, W, 1, 1);
-  else if (kk + 1 <= 3 * kTileR) tile_row<3>(P, V, W, 1, 1);
-  else tile_row<4>(P, V, W, 1, 1);
-  for (int k = 0; k < kk; k++) so[k] = temp[k * NIJ];
-}
 
 // ---- Stokes drift onto the model layers (hyb_stokes.h) ----
 #include "hyb_stokes.h"

@@ -9,10 +9,9 @@ import hybgen_abi as abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SO = os.path.join(ROOT, "tests", "hostcheck", "libhostcheck.so")
-FAMILIES = {"lane-arrays": "hostcheck_hybgen", "register-resident": "hostcheck_hybgen_fast",
+FAMILIES = {"lane-arrays": "hostcheck_hybgen",
             "streaming": "hostcheck_hybgen_stream",
-            "streaming-recompute": "hostcheck_hybgen_stream_rc",
-            "streaming-tiled": "hostcheck_hybgen_stream_tiled"}
+            "streaming-recompute": "hostcheck_hybgen_stream_rc"}
 _lib = None
 
 
@@ -25,7 +24,6 @@ def lib():
                                        ctypes.POINTER(abi.HybFields),
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
         h.hostcheck_stream_unsafe_columns.restype = ctypes.c_int64
-        h.hostcheck_tile_column.restype = None
         h.hostcheck_stokes.argtypes = [ctypes.POINTER(abi.HybStokesParams),
                                        ctypes.POINTER(abi.HybStokesFields), ctypes.c_int]
         h.hostcheck_trc.argtypes = [ctypes.POINTER(abi.HybTrcParams),
@@ -37,9 +35,7 @@ def lib():
 
 def covers(family, P):
     ok = P.ldenfix == 0 and P.nhybrd == P.kdm
-    if family == "register-resident":
-        return ok and P.kdm in (22, 32, 41)
-    if family in ("streaming", "streaming-recompute", "streaming-tiled"):
+    if family in ("streaming", "streaming-recompute"):
         return ok
     return True
 

@@ -12,22 +12,20 @@ import oracle_lib as ol
 pytestmark = pytest.mark.gpu
 
 
-FAMILIES = {1: "lane-arrays", 2: "register-resident", 3: "streaming"}
+FAMILIES = {1: "lane-arrays", 3: "streaming"}
 
 
-@pytest.fixture(params=[1, 2, 3], ids=list(FAMILIES.values()))
+@pytest.fixture(params=[1, 3], ids=list(FAMILIES.values()))
 def kernel_path(request, gpu):
     """Runs the test once per kernel family.  A family that does not cover
-    the case's parameters (register-resident: kdm in {22,32,41}; both:
-    nhybrd==kdm, no HYBGEN_DENFIX) is replaced by the lane-array kernels."""
+    the case's parameters (streaming: nhybrd==kdm, no HYBGEN_DENFIX) is
+    replaced by the lane-array kernels."""
     yield request.param
     abi.check(abi.load().hycom_hybgen_set_kernel_path(0))
 
 
 def _select(family, P):
     ok = P.ldenfix == 0 and P.nhybrd == P.kdm
-    if family == 2 and not (ok and P.kdm in (22, 32, 41)):
-        family = 1
     if family == 3 and not ok:
         family = 1
     print(f"[kernels] {FAMILIES[family]}", flush=True)
@@ -250,49 +248,6 @@ def test_concurrent_uv_phase(gpu, hybmap, raflag):
         abi.check(lib.hycom_hybgen_tune(4, 0))
         abi.check(lib.hycom_hybgen_set_kernel_path(0))
     _assert_same(want, got, f"concurrent u/v hybmap={hybmap} raflag={raflag}")
-
-
-@pytest.mark.parametrize("kdm", [4, 9, 22, 31, 41, 47, 63])
-def test_tiled_remap(gpu, kdm):
-    """The tiled PPM remap (remap variant 16) with 2, 3 and 4 layers per
-    thread gives the oracle's bits, fallback columns included."""
-    P, w = _case("tiled-remap", idm=90, jdm=57, kdm=kdm, hybmap=2, ntracr=3,
-                 trcflg=[2, 0, 0], nsigma=min(8, kdm), dx00=30.0, dx00f=1.1, dx00x=300.0)
-    _select(3, P)
-    lib = abi.load()
-    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=kdm + 5, extreme=kdm % 2))
-    want = _oracle_out(d0)
-    dd = abi.DeviceDomain(P).upload(d0)
-    abi.check(lib.hycom_hybgen_tune(0, 16))
-    try:
-        dd.hybgen(m=1, n=2)
-        got = dd.download()
-    finally:
-        abi.check(lib.hycom_hybgen_tune(0, 3))
-        abi.check(lib.hycom_hybgen_set_kernel_path(0))
-    _assert_same(want, got, f"tiled remap kdm={kdm}")
-
-
-@pytest.mark.parametrize("variant", [16, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 0, 1, 2, 4, 5, 3])
-def test_streaming_remap_variants(gpu, variant):
-    """Every p-column remap variant (fields per walk, prefetch, new
-    interfaces from LDS or read ahead from p, occupancy target) gives the
-    oracle's bits, fallback columns included."""
-    P, w = _case("remap-variant", idm=90, jdm=57, kdm=41, hybmap=2, ntracr=5,
-                 trcflg=[2, 0, 0, 0, 0], dx00=30.0, dx00f=1.1, dx00x=300.0)
-    _select(3, P)
-    lib = abi.load()
-    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=61, extreme=1))
-    want = _oracle_out(d0)
-    dd = abi.DeviceDomain(P).upload(d0)
-    abi.check(lib.hycom_hybgen_tune(0, variant))
-    try:
-        dd.hybgen(m=1, n=2)
-        got = dd.download()
-    finally:
-        abi.check(lib.hycom_hybgen_tune(0, 3))
-        abi.check(lib.hycom_hybgen_set_kernel_path(0))
-    _assert_same(want, got, f"remap variant {variant}")
 
 
 @pytest.mark.parametrize("kdm,hybmap,extreme", [(41, 2, 1), (22, 3, 0), (64, 1, 1)])
