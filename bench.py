@@ -58,13 +58,41 @@ def sea_columns(dd):
     return int(torch.count_nonzero(ip).item())
 
 
+def physical_cores():
+    """Physical cores among the CPUs this process may run on: distinct
+    (core, socket) pairs of `lscpu -p=cpu,core,socket` whose cpu is in the
+    affinity set (logical CPUs of one core count once)."""
+    import subprocess
+    allowed = os.sched_getaffinity(0)
+    try:
+        out = subprocess.run(["lscpu", "-p=cpu,core,socket"], capture_output=True, text=True,
+                             check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return len(allowed), len(allowed)
+    cores = set()
+    for line in out.splitlines():
+        if not line or line.startswith("#"):
+            continue
+        cpu, core, sock = (int(x) if x else -1 for x in line.split(",")[:3])
+        if cpu in allowed:
+            cores.add((core, sock))
+    return (len(cores) or len(allowed)), len(allowed)
+
+
 def cpu_baseline(workload, hybmap, rows, seed):
     """Reference hybgen (flang build of the HYCOM-src sources) on a window of
-    `rows` full-width j-rows of the same global field, on this host."""
+    `rows` full-width j-rows of the same global field, on this host, with
+    one OpenMP thread per physical core of the affinity set."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     if not ol.ref_available():
         return None
+    cores, logical = physical_cores()
+    lib = ol.ref(False)
+    has_omp = hasattr(lib, "omp_set_num_threads") and hasattr(lib, "omp_get_max_threads")
+    nmax = lib.omp_get_max_threads() if has_omp else 0
+    if has_omp:
+        lib.omp_set_num_threads(cores)
     w0 = abi.WORKLOADS[workload]
     itdm, jtdm = w0["idm"], w0["jdm"]
     j0 = max(0, jtdm // 2 - rows // 2)
@@ -77,12 +105,10 @@ def cpu_baseline(workload, hybmap, rows, seed):
         times.append(ol.run_ref(d.copy()))
     times.sort()
     t = times[len(times) // 2]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    threads = cores if has_omp else logical
     # the same reference on one thread, on the first rows // 8 rows of the band
     one = None
-    lib = ol.ref(False)
-    if hasattr(lib, "omp_set_num_threads") and hasattr(lib, "omp_get_max_threads"):
-        nmax = lib.omp_get_max_threads()
+    if has_omp:
         P1, w1 = abi.workload_params(workload, hybmap=hybmap, jdm=max(rows // 8, 1))
         d1 = abi.HostDomain(P1).synth(abi.make_synth(P1, w1, seed=seed, itdm=itdm, jtdm=jtdm,
                                                      i0=0, j0=j0))
@@ -98,10 +124,12 @@ def cpu_baseline(workload, hybmap, rows, seed):
         "value": sea * P.kdm / t,
         "unit": "column-layers/s",
         "cores": threads,
+        "logical_cpus": logical,
         "kind": "reference",
         "sample": (f"HYCOM-src hybgen (flang -O2, OpenMP over j) on rows {j0 + 1}-{j0 + rows} "
                    f"of the {itdm}x{jtdm}x{P.kdm} field ({sea} sea columns), median of 3 "
-                   f"calls of {t:.3f} s"),
+                   f"calls of {t:.3f} s, {threads} OpenMP threads = physical cores of the "
+                   f"{logical} CPUs this process may use (lscpu)"),
         "one_thread": one,
     }
 
@@ -536,6 +564,24 @@ def main():
     total = t1 - t0
     ms_step = 1e3 * total / args.steps
     value = sea * P.kdm * args.steps / total
+
+    # The timed calls above regrid the state earlier calls left, as
+    # successive model steps do.  The first call on freshly generated input
+    # (perturbed isopycnal interfaces, far from the target grid) moves more
+    # interfaces and sends more columns down the branches of the regrid:
+    # timed alone here, three times, the generator outside the HIP events.
+    first = []
+    for q in range(3):
+        dd.synth(S, stream=stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dd.hybgen(m=1, n=2, stream=stream)
+        e1.record()
+        torch.cuda.synchronize()
+        first.append(e0.elapsed_time(e1))
+    first.sort()
+    print(f"[bench] first call on fresh input {first[1]:.2f} ms", file=sys.stderr, flush=True)
     if cnt[7]:
         # the streaming kernels: the p-column remap kernel on its own
         kname = f"{REMAP_KERNELS[args.remap_variant].format(m=args.hybmap)} (p-column remap, hybgenaj)"
@@ -623,6 +669,10 @@ def main():
             "hybgen_raflag": False,
         },
         "latency_ms": {"p50": p50, "p99": p99},
+        "first_call_ms": {"median": first[1], "all": first,
+                          "about": "one call on freshly generated input (the configured "
+                                   "perturbed-isopycnal state), median of 3; the timed steps "
+                                   "regrid the state earlier calls left"},
         "phase_ms": {PHASES[q]: ms[q] / cnt[q] for q in range(np_) if cnt[q]},
         "roofline": {
             "bound": "hbm",

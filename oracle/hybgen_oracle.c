@@ -6,7 +6,7 @@
  * the single-process xctilr halo (mod_xc_sm.h:1337-1429).  Statement order,
  * association and operation order follow the Fortran text so that, built
  * with -O2 -ffp-contract=off on x86-64, every output is bit-identical to the
- * reference compiled with flang (checked by tests/test_oracle_vs_ref.py).
+ * reference compiled with flang (checked by tests/test_oracle_ref.py).
  * Arrays are indexed 1-based like the Fortran.
  */
 #include <math.h>
@@ -733,6 +733,7 @@ typedef struct {
   const hyb_fields* F;
   long NI, NJ, NIJ;
   int K, nb;
+  int32_t *rec_fixlay, *rec_kp; /* optional per-column decisions (idm*jdm, i fastest) */
 } ctx_t;
 
 #define IJ(i, j) ((long)((i) - 1 + c->nb) + c->NI * (long)((j) - 1 + c->nb))
@@ -1008,6 +1009,14 @@ static void hybgenaj(const ctx_t* c, int m, int n, int j) {
           }
         }
       }
+    }
+
+    /* the column's decisions: fixed layers (479-573) and the deepest
+       inflated layer after the overturn (590-686) */
+    if (c->rec_fixlay) {
+      long r = (long)(i - 1) + (long)P->idm * (j - 1);
+      c->rec_fixlay[r] = fixlay;
+      c->rec_kp[r] = kp;
     }
 
     /* massless or near-massless layers, 972-1009 */
@@ -1478,12 +1487,14 @@ static int check(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
 
 /* hybgen, hybgen.F90:14-306 */
 static int run(const hyb_params* P, const hyb_fields* F, int m, int n,
-               int raflag, int nthreads) {
+               int raflag, int nthreads, int32_t* rec_fixlay, int32_t* rec_kp) {
   int rc = check(P, F, m, n, raflag);
   if (rc) return rc;
   eos_init();
   ctx_t cc, *c = &cc;
   c->P = P; c->F = F;
+  c->rec_fixlay = rec_fixlay;
+  c->rec_kp = rec_kp;
   c->NI = P->idm + 2 * P->nbdy;
   c->NJ = P->jdm + 2 * P->nbdy;
   c->NIJ = c->NI * c->NJ;
@@ -1543,10 +1554,16 @@ static int run(const hyb_params* P, const hyb_fields* F, int m, int n,
 }
 
 int oracle_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag) {
-  return run(P, F, m, n, raflag, 1);
+  return run(P, F, m, n, raflag, 1, NULL, NULL);
 }
 
 int oracle_hybgen_omp(const hyb_params* P, const hyb_fields* F, int m, int n,
                       int raflag, int nthreads) {
-  return run(P, F, m, n, raflag, nthreads < 1 ? 1 : nthreads);
+  return run(P, F, m, n, raflag, nthreads < 1 ? 1 : nthreads, NULL, NULL);
+}
+
+int oracle_hybgen_decisions(const hyb_params* P, const hyb_fields* F, int m, int n,
+                            int raflag, int nthreads, int32_t* fixlay, int32_t* kp) {
+  if (!fixlay || !kp) return HYB_ENULL;
+  return run(P, F, m, n, raflag, nthreads < 1 ? 1 : nthreads, fixlay, kp);
 }

@@ -9,7 +9,7 @@
  *
  * Pinning: every function here is compared bit-for-bit against the
  * reference Fortran compiled from its own sources by oracle/ref/Makefile
- * (outputs in oracle/_ref/, see tests/test_oracle_vs_ref.py), and against
+ * (outputs in oracle/_ref/, see tests/test_oracle_ref.py), and against
  * committed golden vectors written by that build (tests/golden/).
  */
 #ifndef HYBGEN_ORACLE_H
@@ -31,6 +31,13 @@ int oracle_hybgen(const hyb_params* P, const hyb_fields* F,
  * reference's !$OMP PARALLEL DO over j does (hybgen.F90:89-95). */
 int oracle_hybgen_omp(const hyb_params* P, const hyb_fields* F,
                       int m, int n, int raflag, int nthreads);
+
+/* oracle_hybgen_omp that also writes each sea p-column's decisions of
+ * hybgenaj: fixlay, the number of fixed layers (hybgen.F90:479-573), and
+ * kp, the deepest inflated layer after the overturn (590-686), at
+ * (i-1) + idm*(j-1); land points are left as they are. */
+int oracle_hybgen_decisions(const hyb_params* P, const hyb_fields* F, int m, int n,
+                            int raflag, int nthreads, int32_t* fixlay, int32_t* kp);
 
 /* sigma-2, 17-term EOS (stmt_fns.h:505-511). */
 double oracle_sig(double t, double s);

@@ -26,6 +26,9 @@ def oracle():
         lib.oracle_hybgen_omp.argtypes = [_P(abi.HybParams), _P(abi.HybFields),
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int]
+        lib.oracle_hybgen_decisions.argtypes = [_P(abi.HybParams), _P(abi.HybFields),
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         lib.oracle_sig.restype = ctypes.c_double
         lib.oracle_sig.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.oracle_remap_column.argtypes = [
@@ -86,6 +89,21 @@ def run_oracle(dom, m=1, n=2, raflag=0, threads=1):
     if rc != 0:
         raise RuntimeError(f"oracle_hybgen rc={rc}")
     return dom
+
+
+def run_oracle_decisions(dom, m=1, n=2, raflag=0, threads=1):
+    """run_oracle that also returns the per-column decisions of hybgenaj:
+    (fixlay, kp) as (idm, jdm) int32 arrays, 0 on land."""
+    import numpy as np
+    P = dom.P
+    fixlay = np.zeros((P.idm, P.jdm), dtype=np.int32, order="F")
+    kp = np.zeros((P.idm, P.jdm), dtype=np.int32, order="F")
+    F = dom.fields()
+    rc = oracle().oracle_hybgen_decisions(ctypes.byref(P), ctypes.byref(F), m, n, raflag,
+                                          threads, fixlay.ctypes.data, kp.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_hybgen_decisions rc={rc}")
+    return fixlay, kp
 
 
 def run_ref(dom, m=1, n=2, raflag=0):

@@ -116,3 +116,50 @@ def test_batch_of_one_equals_device_call(gpu):
     b.hybgen(m=1, n=2)
     torch.cuda.synchronize()
     assert not _same_on_device(a, b)
+
+
+def test_batch_of_config3_shape_matches_oracle(gpu):
+    """The configs[2] batch shape: 64 regional domains of random shape
+    (capped at 300x200 so the oracle finishes in seconds), kdm drawn from
+    {22, 32, 41}, shelf fraction 0-60% (vanished layers), all four
+    remappers, FP64, in one hycom_hybgen_batch call; every domain is
+    compared with the oracle bitwise.  The domains are generated on the
+    device and again on the host, one at a time, for the check (the two
+    generators give the same bits)."""
+    import torch
+    rng = np.random.default_rng(2024)
+    specs = []
+    for d in range(64):
+        specs.append(dict(idm=int(rng.integers(100, 301)), jdm=int(rng.integers(100, 201)),
+                          kdm=int(rng.choice([22, 32, 41])), shelf=float(rng.uniform(0.0, 0.6)),
+                          hybmap=int(rng.integers(0, 4)), nreg=int(rng.choice([0, 1])),
+                          seed=300000 + 100 * d))
+    print(f"\n[case] config-3 shaped batch: {len(specs)} domains "
+          f"{[(s['idm'], s['jdm'], s['kdm']) for s in specs]}", flush=True)
+    doms, synths = [], []
+    for s in specs:
+        P, w = abi.workload_params("cfg1", idm=s["idm"], jdm=s["jdm"], kdm=s["kdm"],
+                                   hybmap=s["hybmap"], nreg=s["nreg"], shelf=s["shelf"],
+                                   nsigma=8)
+        S = abi.make_synth(P, w, seed=s["seed"])
+        doms.append(abi.DeviceDomain(P, skip=("dpo",)).synth(S))
+        synths.append(S)
+    torch.cuda.synchronize()
+    abi.check(abi.load().hycom_hybgen_batch_autotune(0))
+    try:
+        abi.Batch(doms).hybgen(m=1, n=2)
+        torch.cuda.synchronize()
+    finally:
+        abi.check(abi.load().hycom_hybgen_batch_autotune(1))
+    shelf_cols = 0
+    for s, dd, S in zip(specs, doms, synths):
+        want = abi.HostDomain(dd.P).synth(S)
+        nb = dd.P.nbdy
+        ip = want.a["ip"][nb:nb + dd.P.ii, nb:nb + dd.P.jj] > 0
+        shelf_cols += int((want.a["depths"][nb:nb + dd.P.ii, nb:nb + dd.P.jj][ip] < 200.0).sum())
+        ol.run_oracle(want, m=1, n=2, threads=8)
+        got = dd.download([n for n in abi.OUTPUTS])
+        got = abi.HostDomain(dd.P, {**{k: v for k, v in want.a.items()}, **got})
+        df = ol.diff_domains(want, got)
+        assert not df, f"domain {s} differs from the oracle: {df}"
+    assert shelf_cols > 10000  # shelf-heavy columns (shallow, vanished layers) are in the set

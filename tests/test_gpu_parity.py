@@ -116,9 +116,12 @@ def test_device_synth_matches_host_synth(gpu):
     assert not df, f"generator differs host vs device: {df}"
 
 
-def test_larger_domain_matches_oracle(gpu, kernel_path):
+@pytest.mark.parametrize("hybmap", [2, 3], ids=["ppm", "weno"])
+def test_larger_domain_matches_oracle(gpu, kernel_path, hybmap):
+    """A 600x400 window of the GLBb field (kdm 41, 4 tracers) with both
+    remappers the GLBb workload names, PPM and WENO."""
     P, w = _case("mid-size", idm=600, jdm=400, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0],
-                 workload="glbb")
+                 workload="glbb", hybmap=hybmap)
     _select(kernel_path, P)
     S = abi.make_synth(P, w, seed=200, itdm=4500, jtdm=3298, i0=1800, j0=1200)
     dd = abi.DeviceDomain(P).synth(S)
@@ -126,7 +129,7 @@ def test_larger_domain_matches_oracle(gpu, kernel_path):
     want = _oracle_out(d0)
     dd.hybgen(m=1, n=2)
     got = dd.download()
-    _assert_same(want, got, "600x400x41")
+    _assert_same(want, got, f"600x400x41 hybmap={hybmap}")
 
 
 @pytest.mark.parametrize("kdm,hybmap,ntracr,extreme", [
@@ -302,3 +305,91 @@ def test_finite_scan(gpu):
         assert f"i={i0 + 1} j={j0 + 1}" in str(e.value), str(e.value)
         torch.cuda.synchronize()
         assert not ol.diff_domains(bad, dd.download()), "the scan must stop the call"
+
+
+def _colint(h, f, tr=None):
+    P = h.P
+    nb = P.nbdy
+    x = h.a[f][..., 1] if tr is None else h.a[f][..., 1, tr]
+    return (h.a["dp"][..., 1] * x).sum(axis=2)[nb:nb + P.ii, nb:nb + P.jj]
+
+
+@pytest.mark.parametrize("hybmap", [1, 2, 3])
+def test_mixed_on_the_cfg1_shape_with_extreme_inputs(gpu, hybmap):
+    """HYB_PREC_MIXED on the exact configs[3] shape, 150x100x32 with 2
+    tracers, extreme inputs (1e-3 m layers, 40 C / 40 psu fronts), against
+    the FP64 oracle: per field max and RMS error at most 1e-5 of the field's
+    scale, interfaces bitwise, and at every sea column the column integrals
+    sum_k dp*x of T, S and each tracer within 1e-12 (relative) of the
+    input's and of the oracle's."""
+    P, w = _case("mixed-cfg1-extreme", hybmap=hybmap)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=100, extreme=1))
+    want = _oracle_out(d0)
+    P.precision = abi.PREC_MIXED
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2)
+    got = dd.download()
+    for f in ("dp", "p", "dpu", "dpv", "pu", "pv"):
+        assert np.array_equal(want.a[f], got.a[f]), f"{f}: the regrid must stay FP64"
+    for f in ("temp", "saln", "th3d", "tracer", "u", "v"):
+        x, y = want.a[f], got.a[f]
+        scale = np.max(np.abs(x))
+        mx = np.max(np.abs(x - y)) / scale
+        rms = np.sqrt(np.mean((x - y) ** 2)) / scale
+        print(f"[mixed] {f}: max {mx:.2e} rms {rms:.2e}", flush=True)
+        assert mx <= 1e-5 and rms <= 1e-5, f"{f}: max {mx:.2e} rms {rms:.2e}"
+    nb = P.nbdy
+    sea = d0.a["ip"][nb:nb + P.ii, nb:nb + P.jj] > 0
+    for f, tr in [("temp", None), ("saln", None)] + [("tracer", q) for q in range(P.ntracr)]:
+        i0 = _colint(d0, f, tr)
+        den = np.maximum(np.abs(i0), 1e-300)
+        r_in = (np.abs(_colint(got, f, tr) - i0) / den)[sea].max()
+        r_64 = (np.abs(_colint(got, f, tr) - _colint(want, f, tr)) / den)[sea].max()
+        print(f"[mixed] {f}{'' if tr is None else tr + 1}: column residual {r_in:.2e}, "
+              f"vs oracle {r_64:.2e}", flush=True)
+        assert r_in <= 1e-12 and r_64 <= 1e-12, f"{f}{tr}: {r_in:.2e} {r_64:.2e}"
+
+
+@pytest.mark.parametrize("case", [
+    dict(),
+    dict(idm=97, jdm=61, kdm=41, ntracr=3, trcflg=[2, 0, 0], hybmap=3, dx00=30.0, dx00f=1.1,
+         dx00x=300.0, extreme=1),
+    dict(idm=97, jdm=61, hybthk=0.5, hybthn=5.0, nsigma=0, hybrlx=1.0),
+    dict(idm=90, jdm=57, ldenfix=1),
+    dict(idm=90, jdm=57, kdm=41, nhybrd=35, hybiso=0.01),
+], ids=["cfg1", "extreme-dx0k", "hybthk", "denfix", "nhybrd<kdm"])
+def test_decisions_match_oracle(gpu, kernel_path, case):
+    """hycom_hybgen_record_decisions: each sea p-column's fixlay (the fixed
+    layers, hybgen.F90:479-573, HYBGEN_DENFIX included) and kp (the deepest
+    inflated layer after the overturn, 590-686) from the GPU equal the
+    oracle's, column by column."""
+    import torch
+    case = dict(case)
+    extreme = case.pop("extreme", 0)
+    P, w = _case("decisions", **case)
+    _select(kernel_path, P)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=44, extreme=extreme))
+    want = d0.copy()
+    fx_o, kp_o = ol.run_oracle_decisions(want, threads=8)
+    lib = abi.load()
+    fx = torch.zeros(P.idm * P.jdm, dtype=torch.int32, device="cuda")
+    kp = torch.zeros_like(fx)
+    dd = abi.DeviceDomain(P).upload(d0)
+    abi.check(lib.hycom_hybgen_record_decisions(ctypes.c_void_p(fx.data_ptr()),
+                                                ctypes.c_void_p(kp.data_ptr())))
+    try:
+        dd.hybgen(m=1, n=2)
+        torch.cuda.synchronize()
+    finally:
+        abi.check(lib.hycom_hybgen_record_decisions(None, None))
+        abi.check(lib.hycom_hybgen_set_kernel_path(0))
+    fx_g = fx.cpu().numpy().reshape(P.idm, P.jdm, order="F")
+    kp_g = kp.cpu().numpy().reshape(P.idm, P.jdm, order="F")
+    nb = P.nbdy
+    sea = d0.a["ip"][nb:nb + P.ii, nb:nb + P.jj] > 0
+    assert sea.sum() > 100
+    assert np.array_equal(fx_g[sea], fx_o[sea]), f"fixlay differs at {int((fx_g != fx_o)[sea].sum())} columns"
+    assert np.array_equal(kp_g[sea], kp_o[sea]), f"kp differs at {int((kp_g != kp_o)[sea].sum())} columns"
+    # the decisions vary over the domain, so equality is not vacuous
+    assert len(np.unique(fx_o[sea])) > 1 and len(np.unique(kp_o[sea])) > 1
+    _assert_same(want, dd.download(), "outputs beside the decisions")

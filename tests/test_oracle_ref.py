@@ -88,3 +88,31 @@ def test_eos_matches_reference_on_grid():
         for s in np.concatenate([np.linspace(-1.0, 42.0, 53), [0.0, 1e-300]]):
             a, b = o.oracle_sig(t, s), r.href_sig(t, s)
             assert a == b or (np.isnan(a) and np.isnan(b)), (t, s, a, b)
+
+
+@pytest.mark.parametrize("seed,hybmap", [(1, 2), (2, 3), (3, 0)])
+def test_oracle_fixed_layers_agree_with_reference_output(seed, hybmap):
+    """The oracle's exported fixlay (hybgen.F90:479-573) against the
+    reference's own output: where qdep = 1 (nsigma = 0 here) and no maximum
+    thickness applies, the reference keeps interface k+1 at
+    min(dp0cum(k+1), p(kk+1)) for every k <= fixlay (1092-1097), dp0cum
+    being the running sum of dp0k (468).  An oracle that counts one fixed
+    layer too many fails at that level.  The oracle's outputs beside its
+    decisions are the reference's bits."""
+    P, w = abi.workload_params("cfg1", idm=60, jdm=40, nsigma=0, hybmap=hybmap)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=seed))
+    ref = d0.copy()
+    ol.run_ref(ref)
+    orc = d0.copy()
+    fx, _ = ol.run_oracle_decisions(orc, threads=4)
+    assert not ol.diff_domains(ref, orc)
+    K, nb = P.kdm, P.nbdy
+    dp0cum = np.concatenate([[0.0], np.cumsum(np.array(P.dp0k[:K]))])  # dp0cum(1..K+1)
+    p = ref.a["p"][nb:nb + P.ii, nb:nb + P.jj, :]
+    sea = d0.a["ip"][nb:nb + P.ii, nb:nb + P.jj] > 0
+    checked = 0
+    for i, j in np.argwhere(sea):
+        for k in range(1, min(int(fx[i, j]), K - 1) + 1):
+            assert p[i, j, k] == min(dp0cum[k], p[i, j, K]), (i, j, k, int(fx[i, j]))
+            checked += 1
+    assert checked > 10 * sea.sum()
