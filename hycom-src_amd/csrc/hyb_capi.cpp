@@ -714,6 +714,113 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Workspace of hycom_hybgen_device_noalloc: one fixed device array per GPU,
+// held from the library's load until the process ends.  The array's name
+// is only a host-side handle; the device address of each GPU's copy comes
+// from hipGetSymbolAddress with that GPU current, and only that address is
+// ever handed to kernels.
+constexpr size_t kNoallocBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[64] = {};
+
+size_t noalloc_round(size_t b) { return (b + 255) & ~size_t(255); }
+
+// bytes of pres, kp, lcm and uvf for NIJ points and K layers, each piece
+// 256-byte aligned
+size_t noalloc_need(int64_t nij, int K) {
+  return noalloc_round(sizeof(double) * (size_t)nij * (size_t)(K + 1)) +
+         noalloc_round(sizeof(int32_t) * (size_t)nij) +
+         noalloc_round(sizeof(uint64_t) * (size_t)nij) +
+         noalloc_round(sizeof(int32_t) * 2 * (size_t)nij);
+}
+
+// the current device's copy of scratch, looked up once per device
+int noalloc_base(unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(HYB_EDEVICE, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(HYB_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = (unsigned char*)g_noalloc_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t hycom_hybgen_noalloc_capacity(void) { return kNoallocBytes; }
+
+size_t hycom_hybgen_noalloc_need(const hyb_params* P) {
+  if (!P) return 0;
+  const Shape s = shape_of(P);
+  return noalloc_need(s.NIJ, s.K);
+}
+
+int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m, int n,
+                                int raflag, void* stream) {
+  int rc = hycom_hybgen_check_params(P);
+  if (rc) return rc;
+  if ((rc = check_fields(P, F, raflag))) return rc;
+  if (m < 1 || m > 2 || n < 1 || n > 2 || (raflag && m == n))
+    return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
+  if (P->check_finite)
+    return fail(HYB_EUNSUPPORTED,
+                "check_finite needs a result buffer and a host wait; use hycom_hybgen_device");
+  if (P->precision == HYB_PREC_MIXED && !stream_path_ok(kparams_of(P)))
+    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm and no HYBGEN_DENFIX");
+  if (P->precision == HYB_PREC_MIXED && g_kernel_path == 1)
+    return fail(HYB_EUNSUPPORTED, "the mixed form runs on the streaming kernels only");
+  KParams K = kparams_of(P);
+  const int path = g_kernel_path;
+  if (path == 3 && !stream_path_ok(K))
+    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
+  const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
+  const int64_t nij = shape_of(P).NIJ;
+  const size_t need = noalloc_need(nij, P->kdm);
+  if (use_stream && need > kNoallocBytes)
+    return fail(HYB_EINVAL,
+                "domain of %lld points x %d layers needs %zu bytes of workspace; the fixed "
+                "array holds %zu (hycom_hybgen_noalloc_capacity)",
+                (long long)nij, P->kdm, need, kNoallocBytes);
+  if ((rc = device_ready())) return rc;
+  View V = view_of(P, F);
+  V.rec_fixlay = g_rec_fixlay;
+  V.rec_kp = g_rec_kp;
+  if (!use_stream) {  // the per-lane-array kernels use no workspace
+    HIPCHK(launch_hybgen_fp64(K, V, m, n, raflag, (hipStream_t)stream, nullptr));
+    return 0;
+  }
+  unsigned char* base = nullptr;
+  if ((rc = noalloc_base(&base))) return rc;
+  Scratch W;
+  W.pres = (double*)base;
+  base += noalloc_round(sizeof(double) * (size_t)V.NIJ * (size_t)(P->kdm + 1));
+  W.kp = (int32_t*)base;
+  base += noalloc_round(sizeof(int32_t) * (size_t)V.NIJ);
+  W.lcm = (uint64_t*)base;
+  base += noalloc_round(sizeof(uint64_t) * (size_t)V.NIJ);
+  W.uvf = (int32_t*)base;
+  // the u/v phase stays on the caller's stream: a second stream would be
+  // created on first use
+  StreamTuning tune = g_tune;
+  tune.concurrent_uv = 0;
+  HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, tune, (hipStream_t)stream, nullptr));
+  return 0;
+}
+
 int hycom_hybgen_set_kernel_path(int path) {
   if (path < 0 || path > 3) return fail(HYB_EINVAL, "kernel path %d", path);
   if (path == 2) return fail(HYB_EUNSUPPORTED, "kernel path 2 (register-resident) was removed");

@@ -126,6 +126,10 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_device": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "hycom_hybgen_device_noalloc": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "hycom_hybgen_noalloc_capacity": (ctypes.c_size_t, []),
+    "hycom_hybgen_noalloc_need": (ctypes.c_size_t, [_P(HybParams)]),
     "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "hycom_hybgen_set_kernel_path": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
@@ -391,10 +395,10 @@ class DeviceDomain:
                                                ctypes.byref(F), m, n, stream))
         return self
 
-    def hybgen(self, m=1, n=2, raflag=0, stream=None):
+    def hybgen(self, m=1, n=2, raflag=0, stream=None, noalloc=False):
         F = self.fields()
-        check(load().hycom_hybgen_device(ctypes.byref(self.P), ctypes.byref(F), m, n,
-                                         raflag, stream))
+        fn = load().hycom_hybgen_device_noalloc if noalloc else load().hycom_hybgen_device
+        check(fn(ctypes.byref(self.P), ctypes.byref(F), m, n, raflag, stream))
         return self
 
 

@@ -27,6 +27,7 @@
 #ifndef HYCOM_HYBGEN_H
 #define HYCOM_HYBGEN_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -169,6 +170,26 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F,
  * stream).  Returns after enqueueing; the caller synchronises. */
 int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
                         int m, int n, int raflag, void* stream);
+
+/* hycom_hybgen_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The streaming kernels' workspace (old interfaces,
+ * kp, PCM mask and u/v fallback flags: hycom_hybgen_noalloc_need() bytes,
+ * NIJ*(8*(kdm+1) + 20) plus alignment) comes from one fixed device array
+ * of hycom_hybgen_noalloc_capacity() bytes (256 MiB) per GPU, held from the
+ * library's load until the process ends; larger domains fail with
+ * HYB_EINVAL.  Nothing of it needs to start at zero: every word a kernel
+ * reads is written earlier in the same call.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two streams,
+ * two threads) race and give undefined results.  check_finite is refused
+ * (HYB_EUNSUPPORTED), the u/v phase always runs on the given stream, and
+ * the call records no timing events.  Results are bitwise those of
+ * hycom_hybgen_device. */
+int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F,
+                                int m, int n, int raflag, void* stream);
+size_t hycom_hybgen_noalloc_capacity(void);
+size_t hycom_hybgen_noalloc_need(const hyb_params* P);
 
 /* Per-column decisions of the last hycom_hybgen_device() call on this
  * thread's library context, for decision-parity checks: fixlay and kp of

@@ -103,6 +103,39 @@ def test_device_entry_fails_without_gpu():
     assert (d.a["temp"] == before).all()  # nothing computed on the CPU
 
 
+def test_noalloc_entry_sizes_and_no_gpu():
+    """The allocation-free entry point: its workspace size, the refusal of
+    domains beyond the fixed array (checked on the host before any device
+    work) and no CPU fallback."""
+    lib = abi.load()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is visible")
+    except Exception:
+        pass
+    cap = lib.hycom_hybgen_noalloc_capacity()
+    assert cap == 256 << 20
+    P = _params(idm=16, jdm=12)
+    NIJ = (16 + 12) * (12 + 12)
+    r = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    want = r(8 * NIJ * (P.kdm + 1)) + r(4 * NIJ) + r(8 * NIJ) + r(8 * NIJ)
+    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(P)) == want
+    d = abi.HostDomain(P)
+    F = d.fields()
+    assert lib.hycom_hybgen_device_noalloc(ctypes.byref(P), ctypes.byref(F), 1, 2, 0,
+                                           None) == abi.HYB_ENODEVICE
+    # a domain whose workspace exceeds the array: refused by its shape alone
+    big = _params(idm=1200, jdm=900, kdm=41)
+    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(big)) > cap
+    assert lib.hycom_hybgen_device_noalloc(ctypes.byref(big), ctypes.byref(F), 1, 2, 0,
+                                           None) == -1  # HYB_EINVAL
+    assert b"fixed array" in lib.hycom_hybgen_last_error()
+    P.check_finite = 1
+    assert lib.hycom_hybgen_device_noalloc(ctypes.byref(P), ctypes.byref(F), 1, 2, 0,
+                                           None) == -2  # HYB_EUNSUPPORTED
+
+
 def _host_batch(specs):
     """hyb_domain array over host-side fields (only validated on the CPU);
     the caller keeps the returned domains alive while D is used."""
