@@ -821,6 +821,17 @@ int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m,
   return 0;
 }
 
+int hycom_hybgen_selftest_quotients(int which, const double* a, const double* b, double* fast,
+                                    double* ieee, int64_t n, void* stream) {
+  if (which < 0 || which > 4) return fail(HYB_EINVAL, "selftest quotient helper %d", which);
+  if (n < 0 || n > ((int64_t)1 << 31)) return fail(HYB_EINVAL, "selftest length %lld", (long long)n);
+  if (n > 0 && (!a || !b || !fast || !ieee)) return fail(HYB_ENULL, "selftest: NULL array");
+  int rc;
+  if ((rc = device_ready())) return rc;
+  HIPCHK(launch_selftest_quot(which, a, b, fast, ieee, n, (hipStream_t)stream));
+  return 0;
+}
+
 int hycom_hybgen_set_kernel_path(int path) {
   if (path < 0 || path > 3) return fail(HYB_EINVAL, "kernel path %d", path);
   if (path == 2) return fail(HYB_EUNSUPPORTED, "kernel path 2 (register-resident) was removed");

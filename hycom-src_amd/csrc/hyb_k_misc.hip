@@ -161,4 +161,36 @@ hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,
   return hipGetLastError();
 }
 
+// The quotient helpers beside the compiler's division, element by element
+// (testing only: the hot kernels inline the same helpers).
+__global__ void k_selftest_quot(int which, const double* a, const double* b, double* fast,
+                                double* ieee, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double x = a[q], y = b[q];
+  double f, e;
+  if (which == 4) {
+    f = div3(x);
+    e = x / 3.0;
+  } else if (which == 2) {
+    f = qrcp(rcp64(y));
+    e = 1.0 / y;
+  } else {
+    const Rcp64 r = rcp64(y);
+    f = which == 0 ? qdiv(x, r) : which == 1 ? qdiv_pos(x, r) : qdiv_any(x, r);
+    e = x / y;
+  }
+  fast[q] = f;
+  ieee[q] = e;
+}
+
+hipError_t launch_selftest_quot(int which, const double* a, const double* b, double* fast,
+                                double* ieee, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_selftest_quot, dim3((unsigned)nb), dim3(256), 0, s, which, a, b, fast,
+                     ieee, n);
+  return hipGetLastError();
+}
+
 }  // namespace hyb

@@ -81,6 +81,12 @@ int scan_blocks(const KParams& P);
 hipError_t launch_scan_finite(const KParams& P, const View& V, int n, int32_t* out,
                               hipStream_t s);
 
+// Self-test of the FP64 quotient helpers of hyb_math.h on the device:
+// fast[q] from the helper `which` (0 qdiv, 1 qdiv_pos, 2 qrcp, 3 qdiv_any,
+// 4 div3), ieee[q] from the compiler's a/b (1/b, a/3) on the same operands.
+hipError_t launch_selftest_quot(int which, const double* a, const double* b, double* fast,
+                                double* ieee, int64_t n, hipStream_t s);
+
 // Enqueues the synthetic-input generator for window (1..ii, 1..jj).
 hipError_t launch_synth(const SynthDomain& D, const View& V, int ii, int jj,
                         int m, int n, hipStream_t s);

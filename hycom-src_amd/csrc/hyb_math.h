@@ -98,6 +98,100 @@ HYB_HD float grcp(float x) { return frcp(x); }
 HYB_HD double gdiv(double a, double b) { return a / b; }
 HYB_HD float gdiv(float a, float b) { return a * frcp(b); }
 
+// ---------------------------------------------------------------------
+// FP64 quotients that are bitwise IEEE a/b on a bounded operand range.
+//
+// The method: a reciprocal y of b refined by two Newton-Raphson steps
+// (e = 1 - b*y, y = y + y*e, each one fma), the quotient q0 = a*y, its
+// exact remainder r = a - b*q0 (one fma) and Markstein's correction
+// q = q0 + r*y (one fma; P. Markstein, "Computation of elementary functions
+// on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990).
+// On gfx950 this is the core of the compiler's own correctly rounded
+// division, which wraps it in v_div_scale (operands scaled by 2^+-128 when
+// an exponent is extreme), v_div_fmas (the correction, rescaled) and
+// v_div_fixup (zero, infinite and NaN operands, and the sign).  With
+//   2^-8 <= |b| <= 2^100  and  a == 0 or 2^-918 <= |a| < 2^760
+// (so the exponents differ by less than 768 and a/b is normal), no
+// scaling applies: v_div_scale returns its operand and v_div_fmas is a
+// plain fma, so the sequence below (fixup kept) computes the same bits as
+// the compiler's a/b.  The reciprocal is formed once per divisor and
+// shared by every quotient with that divisor, so a second quotient costs 4
+// instructions instead of 11.
+//
+// On the host the same steps run from the IEEE reciprocal, so the host
+// build of the kernels exercises the same correction and zero handling.
+// ---------------------------------------------------------------------
+struct Rcp64 {
+  double b, y;  // divisor and its refined reciprocal
+};
+// two Newton-Raphson steps from a first reciprocal y0 of b
+HYB_HD Rcp64 rcp64_from(double b, double y0) {
+  double e = fma(-b, y0, 1.0);
+  double y = fma(y0, e, y0);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  return Rcp64{b, y};
+}
+HYB_HD Rcp64 rcp64(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rcp64_from(b, __builtin_amdgcn_rcp(b));  // v_rcp_f64
+#else
+  return rcp64_from(b, 1.0 / b);
+#endif
+}
+// v_div_fixup_f64: special operands, then the sign of a/b on |q|
+HYB_HD double div_fixup(double q, double b, double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_div_fixup(q, b, a);
+#else
+  if (isnan(a) || isnan(b) || a == 0.0 || b == 0.0 || isinf(a) || isinf(b)) return a / b;
+  const double m = fabs(q);
+  return (signbit(a) != signbit(b)) ? -m : m;
+#endif
+}
+// a / r.b for a == 0 or 2^-918 <= |a| < 2^760 (and |b| in range)
+HYB_HD double qdiv(double a, const Rcp64& r) {
+  const double q0 = a * r.y;
+  const double rr = fma(-r.b, q0, a);
+  return div_fixup(fma(rr, r.y, q0), r.b, a);
+}
+// the same for a > 0 (no fixup needed: the quotient is positive and normal)
+HYB_HD double qdiv_pos(double a, const Rcp64& r) {
+  const double q0 = a * r.y;
+  const double rr = fma(-r.b, q0, a);
+  return fma(rr, r.y, q0);
+}
+// 1 / r.b
+HYB_HD double qrcp(const Rcp64& r) {
+  const double rr = fma(-r.b, r.y, 1.0);
+  return fma(rr, r.y, r.y);
+}
+// a / r.b for any a: outside the range above, the IEEE division (a branch
+// the wave skips when no lane takes it)
+HYB_HD double qdiv_any(double a, const Rcp64& r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int e = __builtin_amdgcn_frexp_exp(a);  // 0 for 0, inf and NaN
+#else
+  int e = 0;
+  if (isfinite(a) && a != 0.0) (void)frexp(a, &e);
+#endif
+  if (__builtin_expect((unsigned)(e + 917) > 1676u, 0)) return a / r.b;
+  return qdiv(a, r);
+}
+// x / 3 for x == 0 or 2^-918 <= |x| < 2^760: Markstein's correction with
+// y = RN(1/3) (the divisor is a constant, so its reciprocal is exact to
+// half an ulp and the correction rounds correctly)
+HYB_HD double div3(double x) {
+  constexpr double y = 0x1.5555555555555p-2;
+  const double q0 = x * y;
+  const double rr = fma(-3.0, q0, x);
+  return div_fixup(fma(rr, y, q0), 3.0, x);
+}
+
+// An interface pressure the fast quotients accept: 0 or in [2^-60, 2^60]
+// (so a difference of two is 0 or in [2^-112, 2^61]); NaN is not.
+HYB_HD bool tame_p(double v) { return (v == 0.0) | ((v >= 0x1p-60) & (v <= 0x1p60)); }
+
 // sig(t,s) = sig_n*(1/sig_d) - 1000, stmt_fns.h:505-511
 HYB_HD double sig(double t, double s) {
   double sn = c101 + t * (c002 + t * (c103 + t * c004)) +

@@ -118,6 +118,10 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   double pk1 = pk + FK(dpc, 1);  // p(k+1), k = 1
   pres[0] = 0.0;
   pres[S] = pk1;
+  // The remap's fast quotients need every old interface (after the
+  // overturn) and every new one to be 0 or in [2^-60, 2^60], the old ones
+  // in order; a column where that fails goes to the fallback remap.
+  bool tame = (FK(dpc, 1) >= 0.0) & tame_p(pk1);
   int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
   int kp = 2;
   // dp(k..k+3) loaded ahead: each level has three iterations to arrive
@@ -131,6 +135,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     pk = pk1;
     pk1 = pk + dk;
     pres[(int64_t)k * S] = pk1;
+    tame = tame & (dk >= 0.0) & tame_p(pk1);
     if (k >= 3 && pk1 - pk >= dpthin) kp = k;  // 590-598: the deepest such k
     double dtf = 0.0, cumk1;
     if (tf) {
@@ -270,6 +275,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
       const double p_hat = q * (pkp - pkk);
       const double pknew = pkk + p_hat;
       pres[(int64_t)(k - 1) * S] = pknew;
+      tame = tame & tame_p(pknew) & (pknew >= pkm) & (pknew <= pkp);
       FK(temp, k) = FK(temp, k) + (q / (1.0 - q)) * (FK(temp, k) - FK(temp, k - 1));
       FK(saln, k) = FK(saln, k) + (q / (1.0 - q)) * (FK(saln, k) - FK(saln, k - 1));
       FK(th3d, k) = sig(FK(temp, k), FK(saln, k)) - P.thbase;
@@ -387,6 +393,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     FK(dpc, jx) = d;
     prs = prs + d;
     pg[(int64_t)jx * S] = prs;
+    tame = tame & tame_p(prs);
     pf = v;
     // remap output jx ends in old layer lb <= jx-3, before the old layer
     // that reaches the bottom, only if pres(jx-2) >= zb = min(prsf(jx+1),zx)
@@ -627,7 +634,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   consume(K - 1, pw[1], K - 1 >= 5 ? pres[(int64_t)(K - 5) * S] : 0.0);
   consume(K, pw[2], K >= 5 ? pres[(int64_t)(K - 4) * S] : 0.0);
   consume(K + 1, pw[3], K + 1 >= 5 ? pres[(int64_t)(K - 3) * S] : 0.0);
-  W.kp[c0] = kp | (unsafe ? kUnsafe : 0);
+  W.kp[c0] = kp | ((unsafe || !tame) ? kUnsafe : 0);
   W.lcm[c0] = lcm;
   if (V.rec_fixlay) {
     int64_t r = (int64_t)(i - 1) + (int64_t)V.rec_ld * (j - 1);
@@ -876,16 +883,29 @@ HYB_HD R gsign(R a, R b) {  // Fortran sign(a,b)
   return signbit(b) ? -x : x;
 }
 
-template <int METHOD, int NF, class R, class LCMB>
+// FD (R = double only): the reciprocals and the quotients of thicknesses
+// through rcp64/qrcp/qdiv_pos, bitwise the IEEE ones; every divisor here
+// is a sum of thicknesses >= thin, which holds for columns whose
+// interfaces are in order and bounded (the ones regrid_stream leaves to
+// the walk).
+template <int METHOD, int NF, class R, bool FD, class LCMB>
 __attribute__((always_inline)) HYB_HD void coef_layer(
     int l, int K, int nf, const LCMB& lcmb, bool lc_l, R thin, R dsmll, R dprs_m, R dprs_0,
     R dprs_p, R dpi_m, R dpi_0, R dpi_p, R dpi_p2, const R* s0, const R* s1, const R* s2,
     const R* s3, CoefState<R, NF>& st, R* c1l, R* c2l, R* c3l) {
+  auto RCP = [](R x) -> R {
+    if constexpr (FD) return qrcp(rcp64(x));
+    else return grcp(x);
+  };
+  auto DIV = [](R a, R b) -> R {
+    if constexpr (FD) return qdiv_pos(a, rcp64(b));
+    else return gdiv(a, b);
+  };
   if (METHOD == 1) {
     // hybgen_plm_coefs (2393-2458) on dprs
     const bool flat = (l == 1 || l == K || lc_l || dprs_0 <= thin);
     R qcen = R(0);
-    if (!flat) qcen = gdiv(dprs_0, dprs_0 + R(0.5) * (dprs_m + dprs_p));
+    if (!flat) qcen = DIV(dprs_0, dprs_0 + R(0.5) * (dprs_m + dprs_p));
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       c1l[f] = R(0);
@@ -909,16 +929,16 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
     if (jn <= K - 1) {
       const R dpjp_l = dpi_0 + dpi_p;
       dpjp_n = dpi_p + dpi_p2;
-      qdpjp_n = grcp(dpjp_n);  // qdpjp(l+1)
+      qdpjp_n = RCP(dpjp_n);  // qdpjp(l+1)
       dp2jp_l = dpi_0 + dpjp_l;
       dpj2p_n = dpjp_n + dpi_p2;
-      dpq3 = gdiv(dpi_p, dpi_0 + dpjp_n);
+      dpq3 = DIV(dpi_p, dpi_0 + dpjp_n);
       if (do_al) {
         dpjp_lm = dpi_m + dpi_0;
         const R dpj2p_l = dpjp_l + dpi_p;
-        qdp2jp_l = grcp(dp2jp_l);
-        qdpj2p_l = grcp(dpj2p_l);
-        qdp4 = grcp(dpjp_lm + dpjp_n);
+        qdp2jp_l = RCP(dp2jp_l);
+        qdpj2p_l = RCP(dpj2p_l);
+        qdp4 = RCP(dpjp_lm + dpjp_n);
       }
     }
     const bool lim = l >= 2 && l <= K - 1;
@@ -982,9 +1002,9 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
     const int jn = l + 1;
     const bool first_full = jn >= 2 && jn <= K - 1 && !(lcmb(jn) || dpi_p <= thin);
     R wq_n = R(0), qmjp = R(0), dpjm2jp = R(0);
-    if (jn <= K - 1) wq_n = grcp(dpi_p + dpi_p2);  // qdpjm(l+2)
+    if (jn <= K - 1) wq_n = RCP(dpi_p + dpi_p2);  // qdpjm(l+2)
     if (first_full) {
-      qmjp = grcp(dpi_0 + dpi_p + dpi_p2);
+      qmjp = RCP(dpi_0 + dpi_p + dpi_p2);
       dpjm2jp = dpi_0 + R(2) * dpi_p + dpi_p2;
     }
     const bool second = l >= 2 && l <= K - 1 && !(lc_l || dpi_0 <= thin);
@@ -1068,8 +1088,8 @@ __attribute__((always_inline)) HYB_HD void coef_layer(
 // double, or a value that converts to one when the level enters the window
 // (a blend of two stored words, so the loads stay outstanding until then).
 // ---------------------------------------------------------------------
-template <int METHOD, int NF, int PD, bool FAST = false, bool CDPI = false, class PI, class PO,
-          class SINK, class FP>
+template <int METHOD, int NF, int PD, bool FAST = false, bool CDPI = false, bool FDIV = false,
+          class PI, class PO, class SINK, class FP>
 HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const FP& fp,
                       int nf, int kp, uint64_t lcm, int64_t S, int KO = 0) {
   constexpr int PQ = PD > 0 ? PD : 1;
@@ -1078,6 +1098,11 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   using CR = typename std::conditional<FAST, float, double>::type;
   const double thin = kDpthin;
   const double dsmll = 1.0e-8;
+  // the FP64 quotients through rcp64/qdiv (hyb_math.h): the caller passes
+  // only columns whose interfaces are in order and bounded, so every
+  // divisor is in [thin, 2^63] and every thickness or depth difference is 0
+  // or in [2^-112, 2^61]
+  constexpr bool FD64 = FDIV && !FAST;
   auto lcmb = [&](int l) -> bool { return (lcm >> (l - 1)) & 1ull; };
 
   // per-field state
@@ -1153,8 +1178,13 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   cst.wq_l = 0.0;
   if (K >= 2) {
     const CR d1 = gmax((CR)(pv2 - pv1), (CR)thin), d2 = gmax((CR)(pv3 - pv2), (CR)thin);
-    cst.qdpjp_l = grcp(d1 + d2);
-    cst.wq_l = grcp(d1 + d2);
+    if constexpr (FD64) {
+      cst.qdpjp_l = qrcp(rcp64(d1 + d2));
+      cst.wq_l = cst.qdpjp_l;
+    } else {
+      cst.qdpjp_l = grcp(d1 + d2);
+      cst.wq_l = grcp(d1 + d2);
+    }
   }
 
   // walk state (2729-2735)
@@ -1225,7 +1255,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
     // ------------------------------------------ coefficients of layer l
     double c1l[NF], c2l[NF], c3l[NF];
     if constexpr (!FAST) {
-      coef_layer<METHOD, NF, double>(l, K, nf, lcmb, lc_l, thin, dsmll, CDPI ? dpi_m : dprs_m,
+      coef_layer<METHOD, NF, double, FD64>(l, K, nf, lcmb, lc_l, thin, dsmll, CDPI ? dpi_m : dprs_m,
                                      CDPI ? dpi_0 : dprs_0, CDPI ? dpi_p : dprs_p, dpi_m, dpi_0,
                                      dpi_p, dpi_p2, s0, s1, s2, s3, cst, c1l, c2l, c3l);
     } else {
@@ -1243,7 +1273,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
         sf2[f] = (float)s2[f];
         sf3[f] = (float)s3[f];
       }
-      coef_layer<METHOD, NF, float>(l, K, nf, lcmb, lc_l, thf, 1.0e-8f, fm, f0, fq,
+      coef_layer<METHOD, NF, float, false>(l, K, nf, lcmb, lc_l, thf, 1.0e-8f, fm, f0, fq,
                                     gmax(fm, thf), gmax(f0, thf), gmax(fq, thf), gmax(fq2, thf),
                                     sf0, sf1, sf2, sf3, cst, g1, g2, g3);
 #pragma unroll
@@ -1268,10 +1298,16 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
     const double pil = pv1, pil1 = pv2, dprsl = dprs_0;
     // x of a depth in layer l; the mixed mode multiplies by a reciprocal
     const double rdl = FAST ? frcp64(dmax(dprsl, thin)) : 0.0;
+    // FD64: one reciprocal of the layer's thickness for every x of the layer
+    const Rcp64 rl = FD64 ? rcp64(dmax(dprsl, thin)) : Rcp64{0.0, 0.0};
     auto xof = [&](double z) -> double {
+      if constexpr (FD64) return qdiv(z - pil, rl);
       return FAST ? (z - pil) * rdl : (z - pil) / dmax(dprsl, thin);
     };
-    auto third = [](double x) -> double { return FAST ? x * (1.0 / 3.0) : x / 3.0; };
+    auto third = [](double x) -> double {
+      if constexpr (FD64) return div3(x);
+      return FAST ? x * (1.0 / 3.0) : x / 3.0;
+    };
     // Every output still to come ends in this layer when it reaches the
     // bottom, and the walk stops here: fold the levels not read yet into
     // the column range now, before any of them is overwritten.
@@ -1301,6 +1337,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
           }
           const double dz = zb_act - zt_act;
           const double rdz = FAST ? frcp64(dz) : 0.0;
+          const Rcp64 rz = FD64 ? rcp64(dz) : Rcp64{0.0, 0.0};
 #pragma unroll
           for (int f = 0; f < NF; f++) {
             v[f] = 0.0;
@@ -1317,7 +1354,8 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
               } else {
                 a = a + dpb * (qb0 * (s1[f] - o) + qb1 * (c1l[f] - o) + qb2 * (c2l[f] - o));
               }
-              v[f] = FAST ? o + a * rdz : o + a / dz;
+              if constexpr (FD64) v[f] = o + qdiv_any(a, rz);
+              else v[f] = FAST ? o + a * rdz : o + a / dz;
             }
           }
         }
@@ -1351,6 +1389,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
             const double xb = xof(zb);
             const double dz = zb - zt;
             const double rdz = FAST ? frcp64(dz) : 0.0;
+            const Rcp64 rz = FD64 ? rcp64(dz) : Rcp64{0.0, 0.0};
             double qt0 = 0.0, qt1 = 0.0, qt2 = 0.0;
             if (METHOD == 1) {
               qt1 = (xb * xb - xt * xt - (xb - xt)) * 0.5;
@@ -1368,12 +1407,14 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
               if (f < nf) {
                 if (METHOD == 1) {
                   const double sz = dprsl * (c1l[f] * qt1);
-                  v[f] = FAST ? s1[f] + sz * rdz : s1[f] + sz / dz;
+                  if constexpr (FD64) v[f] = s1[f] + qdiv_any(sz, rz);
+                  else v[f] = FAST ? s1[f] + sz * rdz : s1[f] + sz / dz;
                 } else if (METHOD == 2) {
                   const double oo = s1[f];
                   const double sz =
                       dprsl * ((c1l[f] - oo) * qt0 + (c2l[f] + c3l[f]) * qt1 - c3l[f] * qt2);
-                  v[f] = FAST ? oo + sz * rdz : oo + sz / dz;
+                  if constexpr (FD64) v[f] = oo + qdiv_any(sz, rz);
+                  else v[f] = FAST ? oo + sz * rdz : oo + sz / dz;
                 } else {
                   const double oo = s1[f];
                   const double sz = qt1 * (c1l[f] - oo) + qt2 * (c2l[f] - oo);
@@ -1519,7 +1560,7 @@ HYB_HDI void remap_walk(const KParams& P, const View& V, int64_t c0, int n, int 
   double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
   PiArray pi{pres, S};
   FieldSink<NW, NT> sink{fp, NW, th, P.thbase, S};
-  walk_core<METHOD, NW, PD, FAST>(K, zx, pi, po, sink, fp, NW, kp, lcm, S);
+  walk_core<METHOD, NW, PD, FAST, false, true>(K, zx, pi, po, sink, fp, NW, kp, lcm, S);
 }
 
 // the last walk, over r < NF fields (r = 0: none)
@@ -1642,6 +1683,9 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;  // pres(k-2), pres(k-1), pres(k)
   bool have = false;
   double cand = 0.0;
+  // the walk's fast quotients need the old interfaces in order and both
+  // sets of interfaces 0 or in [2^-60, 2^60]; other points go to point_c
+  bool tame = true;
 #pragma unroll 8
   for (int k = 1; k <= K; k++) {
     const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
@@ -1655,23 +1699,25 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
       cand = pr;
       have = true;
     }
-    const double q3 = q2 + dpu[(int64_t)(k - 1) * S];
+    const double dk = dpu[(int64_t)(k - 1) * S];
+    const double q3 = q2 + dk;
+    tame = tame & (dk >= 0.0) & tame_p(q3) & tame_p(pr);
     q0 = q1;
     q1 = q2;
     q2 = q3;
   }
   const double zx = q2;
-  if (have && cand < zx) return false;
+  if ((have && cand < zx) || !tame) return false;
   PiSum pi{dpu, S};
   double* fp[1] = {uu};
   if (RECOMP) {
     PoRecompute po(p0, p1, S, depthu, K);
     UvSink<PoRecompute> sink{uu, S, ub, pr, &po, 0ull, dpu};
-    walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    walk_core<METHOD, 1, PD, MIXED, false, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
   } else {
     PoSum po{dcol};
     UvSink<PoSum> sink{uu, S, ub, pr, &po};
-    walk_core<METHOD, 1, PD, MIXED>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
+    walk_core<METHOD, 1, PD, MIXED, false, true>(K, zx, pi, po, sink, fp, 1, K, 0ull, S);
 #pragma unroll 2
     for (int k = 1; k <= K; k++) dpu[(int64_t)(k - 1) * S] = dcol(k);
   }

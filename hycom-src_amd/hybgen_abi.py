@@ -131,6 +131,8 @@ SIGNATURES = {
     "hycom_hybgen_noalloc_capacity": (ctypes.c_size_t, []),
     "hycom_hybgen_noalloc_need": (ctypes.c_size_t, [_P(HybParams)]),
     "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "hycom_hybgen_selftest_quotients": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 4
+                                        + [ctypes.c_int64, ctypes.c_void_p]),
     "hycom_hybgen_set_kernel_path": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_timing": (ctypes.c_int, [ctypes.c_int]),

@@ -197,6 +197,15 @@ size_t hycom_hybgen_noalloc_need(const hyb_params* P);
  * pointers; NULL disables recording (the default). */
 int hycom_hybgen_record_decisions(int32_t* fixlay_dev, int32_t* kp_dev);
 
+/* Testing hook (no reference counterpart): the FP64 quotient helpers the
+ * hot kernels use in place of the compiler's division, beside that division,
+ * on n device operands.  which: 0 a/b (qdiv), 1 a/b for a > 0 (qdiv_pos),
+ * 2 1/b (qrcp), 3 a/b for any a (qdiv_any), 4 a/3 (div3).  fast[] gets the
+ * helper's result and ieee[] the compiler's a/b (1/b, a/3).  Device
+ * pointers; enqueued on the stream. */
+int hycom_hybgen_selftest_quotients(int which, const double* a, const double* b,
+                                    double* fast, double* ieee, int64_t n, void* stream);
+
 /* Kernel selection (testing and measurement; no reference counterpart).
  * 0: automatic (streaming kernels when nhybrd==kdm and no HYBGEN_DENFIX,
  * per-lane-array kernels otherwise), 1: per-lane-array kernels only,

@@ -332,3 +332,24 @@ extern "C" int hostcheck_trc(const hyb_trc_params* P, const hyb_trc_fields* F) {
     }
   return bad ? HYB_EINVAL : 0;
 }
+
+// The quotient helpers of hyb_math.h on arrays, from a given first
+// reciprocal y0 (so a test can stand in for the hardware's v_rcp_f64 with
+// reciprocals of a chosen error).  which: 0 qdiv, 1 qdiv_pos, 2 qrcp,
+// 3 qdiv_any, 4 div3 (b and y0 unused).
+extern "C" void hostcheck_quot(int which, const double* a, const double* b, const double* y0,
+                               double* out, int64_t n) {
+  for (int64_t q = 0; q < n; q++) {
+    if (which == 4) {
+      out[q] = div3(a[q]);
+      continue;
+    }
+    const Rcp64 r = rcp64_from(b[q], y0[q]);
+    switch (which) {
+      case 0: out[q] = qdiv(a[q], r); break;
+      case 1: out[q] = qdiv_pos(a[q], r); break;
+      case 2: out[q] = qrcp(r); break;
+      default: out[q] = qdiv_any(a[q], r); break;
+    }
+  }
+}
