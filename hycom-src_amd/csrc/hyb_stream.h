@@ -722,8 +722,12 @@ struct PoAhead {
 
 // The same prsf(k) without the LDS column: the new thicknesses are formed
 // again from p at the two neighbouring p-points as the walk asks for them
-// (the operations of the first pass of uv_stream, so the same bits).
-struct PoRecompute {
+// (the operations of the first pass of uv_stream, so the same bits).  The
+// p levels are loaded A levels ahead of the request, unconditionally (the
+// index clamped into the column), so the loads are issued without a
+// branch and have the work of A requests to arrive.
+template <int A>
+struct PoRecomputeT {
   const double* p0;
   const double* p1;
   int64_t S;
@@ -731,10 +735,13 @@ struct PoRecompute {
   int k1;                                // kk+1, the deepest level
   double prv = 0.0, cur = 0.0, b = 0.0;  // prsf(k-1), prsf(k), min(depthu, p-mean at k)
   double dl = 0.0;                       // the new thickness of layer k-1 (cur = prv + dl)
-  double q0 = 0.0, q1 = 0.0;             // p at level k+1 of both points, loaded ahead
+  double q0[A], q1[A];                   // p at levels k+1..k+A of both points, loaded ahead
   int k = 0;                             // 0 until the first request
-  HYB_HD PoRecompute(const double* a0, const double* a1, int64_t s, double du, int kk)
-      : p0(a0), p1(a1), S(s), depthu(du), k1(kk + 1) {}
+  HYB_HD PoRecomputeT(const double* a0, const double* a1, int64_t s, double du, int kk)
+      : p0(a0), p1(a1), S(s), depthu(du), k1(kk + 1) {
+#pragma unroll
+    for (int q = 0; q < A; q++) q0[q] = q1[q] = 0.0;
+  }
   HYB_HD double operator()(int kq) {
     if (kq == 1 || k == 0) {
       prv = 0.0;
@@ -742,25 +749,39 @@ struct PoRecompute {
       dl = 0.0;
       b = dmin(depthu, 0.5 * (p0[0] + p1[0]));
       k = 1;
-      q0 = p0[S];
-      q1 = p1[S];
+#pragma unroll
+      for (int q = 0; q < A; q++) {
+        const int x = 1 + q < k1 ? 1 + q : k1 - 1;
+        q0[q] = p0[(int64_t)x * S];
+        q1[q] = p1[(int64_t)x * S];
+      }
     }
     while (k < kq) {
-      const double a = dmin(depthu, 0.5 * (q0 + q1));
+      const double a = dmin(depthu, 0.5 * (q0[0] + q1[0]));
       const double d = dmax(0.0, a - b);
       b = a;
       prv = cur;
       cur = cur + d;
       dl = d;
       k++;
-      if (k < k1) {
-        q0 = p0[(int64_t)k * S];
-        q1 = p1[(int64_t)k * S];
+#pragma unroll
+      for (int q = 0; q < A - 1; q++) {
+        q0[q] = q0[q + 1];
+        q1[q] = q1[q + 1];
       }
+      // level k+A-1, clamped into the column (values beyond it unused)
+      const int x = k + A - 1 < k1 ? k + A - 1 : k1 - 1;
+      q0[A - 1] = p0[(int64_t)x * S];
+      q1[A - 1] = p1[(int64_t)x * S];
     }
     return cur;
   }
 };
+using PoRecompute = PoRecomputeT<1>;
+template <class PO>
+struct is_recompute : std::false_type {};
+template <int A>
+struct is_recompute<PoRecomputeT<A>> : std::true_type {};
 
 // Output sinks.  emit(kc, v, certain, thick): v[f] is output kc of field
 // f before the clamp; certain means it lies inside the range of the input
@@ -840,7 +861,7 @@ struct UvSink {  // u or v with the thin-layer rule (hybgen.F90:1858-1865)
     } else {
       uu[(int64_t)(kc - 1) * S] = -ub;
     }
-    if constexpr (std::is_same<PO, PoRecompute>::value) dpn[(int64_t)(kc - 1) * S] = po->dl;
+    if constexpr (is_recompute<PO>::value) dpn[(int64_t)(kc - 1) * S] = po->dl;
   }
   HYB_HD void fixup(const double* lo, const double* hi, const double* smin,
                     const double* smax) {
@@ -1629,6 +1650,24 @@ HYB_HDI StagedInterfaces stage_interfaces(double* row, int ld, const double* pg,
   stage_column(row, ld, pg + S, S, K - 1);
   return StagedInterfaces{row, ld, K + 1, top, bot};
 }
+// The same staged column read one request ahead: a walk asks for prsf(1),
+// then prsf(2), prsf(3), ... each once (a new walk starts again at 1), so
+// the LDS read of level k+1 is issued when level k is handed out and has a
+// whole output's work to arrive.
+struct StagedAhead {
+  StagedInterfaces s;
+  int k = 0;
+  double cur = 0.0, nxt = 0.0;
+  HYB_HD explicit StagedAhead(const StagedInterfaces& si) : s(si) {}
+  HYB_HD double operator()(int kq) {
+    if (kq != k) {
+      cur = (kq == k + 1) ? nxt : s(kq);
+      k = kq;
+      nxt = s(kq + 1);
+    }
+    return cur;
+  }
+};
 
 // ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
@@ -1686,9 +1725,9 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
   // the walk's fast quotients need the old interfaces in order and both
   // sets of interfaces 0 or in [2^-60, 2^60]; other points go to point_c
   bool tame = true;
-#pragma unroll 8
-  for (int k = 1; k <= K; k++) {
-    const double a = dmin(depthu, 0.5 * (p0[(int64_t)k * S] + p1[(int64_t)k * S]));
+  // level k of the pass: p at both points (level k+1) and the old dpu(k)
+  auto step = [&](int k, double pa, double pb, double dk) {
+    const double a = dmin(depthu, 0.5 * (pa + pb));
     const double d = dmax(0.0, a - b);
     b = a;
     if (!RECOMP) dcol(k) = d;
@@ -1699,12 +1738,43 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
       cand = pr;
       have = true;
     }
-    const double dk = dpu[(int64_t)(k - 1) * S];
     const double q3 = q2 + dk;
     tame = tame & (dk >= 0.0) & tame_p(q3) & tame_p(pr);
     q0 = q1;
     q1 = q2;
     q2 = q3;
+  };
+  {
+    // batches of 8 levels, the next batch's loads issued before the
+    // current batch is summed (indices clamped into the column; values
+    // beyond it are not used): a wave waits on memory once per batch with
+    // the next batch already on its way
+    constexpr int B = 8;
+    double ca[B], cb[B], cd[B];
+    auto load = [&](int k0, double* xa, double* xb, double* xd) {
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        const int k = k0 + q <= K ? k0 + q : K;
+        xa[q] = p0[(int64_t)k * S];
+        xb[q] = p1[(int64_t)k * S];
+        xd[q] = dpu[(int64_t)(k - 1) * S];
+      }
+    };
+    load(1, ca, cb, cd);
+#pragma unroll 1
+    for (int k0 = 1; k0 <= K; k0 += B) {
+      double na[B], nb[B], nd[B];
+      load(k0 + B <= K ? k0 + B : K, na, nb, nd);
+#pragma unroll
+      for (int q = 0; q < B; q++)
+        if (k0 + q <= K) step(k0 + q, ca[q], cb[q], cd[q]);
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        ca[q] = na[q];
+        cb[q] = nb[q];
+        cd[q] = nd[q];
+      }
+    }
   }
   const double zx = q2;
   if ((have && cand < zx) || !tame) return false;
