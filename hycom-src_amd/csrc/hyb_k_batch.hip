@@ -95,7 +95,7 @@ kb_regrid(BatchDevice B, BatchGroup G, int n) {
   const int i = r.i0 + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, r.j)] == 0) return;
-  regrid_stream<true>(P, V, gscratch(B.W, r.d), i, r.j, n);
+  regrid_stream<1>(P, V, gscratch(B.W, r.d), i, r.j, n);
 }
 
 // CW columns per workgroup; each lane's LDS column has stride CW doubles

@@ -52,7 +52,7 @@ k_regrid_a(KParams P, View V, Scratch W, int n) {
   int i = i_aligned0(V, j) + blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 1 || i > P.ii) return;
   if (V.ip[ij_of(V, i, j)] == 0) return;
-  regrid_stream<true>(P, V, W, i, j, n);
+  regrid_stream<1>(P, V, W, i, j, n);
 }
 
 template <int METHOD, int PD, bool MIXED = false, bool RECOMP = false>

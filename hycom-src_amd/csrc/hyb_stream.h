@@ -76,8 +76,9 @@ HYB_HD void shl(double (&a)[W], double v) {
 // regrid of one p-column (hybgenaj up to the remap)
 // ---------------------------------------------------------------------
 // AHEAD: the pass-3 window's next levels (pres, th3d, theta) are loaded
-// one layer iteration before the slide takes them in (more registers)
-template <bool AHEAD = false>
+// AHEAD (0, 1 or 2) layer iterations before the slide takes them in (more
+// registers)
+template <int AHEAD = 0>
 HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, int i, int j,
                            int n) {
   const int K = P.kk;
@@ -124,14 +125,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   bool tame = (FK(dpc, 1) >= 0.0) & tame_p(pk1);
   int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
   int kp = 2;
-  // dp(k..k+3) loaded ahead: each level has three iterations to arrive
-  double dq[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) dq[q] = FK(dpc, (2 + q <= K) ? 2 + q : K);
-#pragma unroll 1
-  for (int k = 2; k <= K; k++) {
-    const double dk = dq[0];
-    shl(dq, FK(dpc, (k + 4 <= K) ? k + 4 : K));
+  auto level = [&](int k, double dk) {
     pk = pk1;
     pk1 = pk + dk;
     pres[(int64_t)k * S] = pk1;
@@ -166,6 +160,26 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
       }
     }
     cumk = cumk1;
+  };
+  {
+    // dp of levels 2..K in batches of 8, the next batch loaded before the
+    // current one is scanned (indices clamped into the column; values
+    // beyond it are not used)
+    constexpr int B = 8;
+    double cur[B];
+#pragma unroll
+    for (int q = 0; q < B; q++) cur[q] = FK(dpc, (2 + q <= K) ? 2 + q : K);
+#pragma unroll 1
+    for (int k0 = 2; k0 <= K; k0 += B) {
+      double nxt[B];
+#pragma unroll
+      for (int q = 0; q < B; q++) nxt[q] = FK(dpc, (k0 + B + q <= K) ? k0 + B + q : K);
+#pragma unroll
+      for (int q = 0; q < B; q++)
+        if (k0 + q <= K) level(k0 + q, cur[q]);
+#pragma unroll
+      for (int q = 0; q < B; q++) cur[q] = nxt[q];
+    }
   }
   const double zx = pk1;  // p(kk+1); the overturn does not move it
 
@@ -404,15 +418,20 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   };
 
   // with AHEAD: pres(k+4), th3d(k+3), theta(k+3) for the slide of
-  // iteration k, loaded during iteration k-1 (indices clamped into the
+  // iteration k, loaded during iteration k-AHEAD (indices clamped into the
   // column; values beyond it are not used)
   auto cl1 = [&](int x) -> int64_t { return (int64_t)((x <= K + 1 ? x : K + 1) - 1) * S; };
   auto clk = [&](int x) -> int64_t { return (int64_t)((x <= K ? x : K) - 1) * S; };
-  double nr4 = 0.0, nth3 = 0.0, nta3 = 0.0;
-  if (AHEAD) {
-    nr4 = pres[cl1(6)];
-    nth3 = th3d[clk(5)];
-    nta3 = theta[clk(5)];
+  constexpr int AQ = AHEAD > 0 ? AHEAD : 1;
+  double nr4[AQ], nth3[AQ], nta3[AQ];
+#pragma unroll
+  for (int q = 0; q < AQ; q++) {
+    nr4[q] = nth3[q] = nta3[q] = 0.0;
+    if (AHEAD > q) {
+      nr4[q] = pres[cl1(6 + q)];
+      nth3[q] = th3d[clk(5 + q)];
+      nta3[q] = theta[clk(5 + q)];
+    }
   }
   double q, p_hat, p_hat0, p_hat2, p_hat3;
 #pragma unroll 1
@@ -599,12 +618,18 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     // slide: level k+4 enters pw, k+3 enters tw/thw/lcm, k+2 enters dw/qw
     double r4, th3, ta3;
     if (AHEAD) {
-      r4 = (k + 4 <= K + 1) ? nr4 : 0.0;
-      th3 = (k + 3 <= K) ? (k + 3 > kp ? th_kp : nth3) : 0.0;
-      ta3 = (k + 3 <= K) ? nta3 : 0.0;
-      nr4 = pres[cl1(k + 5)];
-      nth3 = th3d[clk(k + 4)];
-      nta3 = theta[clk(k + 4)];
+      r4 = (k + 4 <= K + 1) ? nr4[0] : 0.0;
+      th3 = (k + 3 <= K) ? (k + 3 > kp ? th_kp : nth3[0]) : 0.0;
+      ta3 = (k + 3 <= K) ? nta3[0] : 0.0;
+#pragma unroll
+      for (int q = 0; q < AQ - 1; q++) {
+        nr4[q] = nr4[q + 1];
+        nth3[q] = nth3[q + 1];
+        nta3[q] = nta3[q + 1];
+      }
+      nr4[AQ - 1] = pres[cl1(k + 4 + AQ)];
+      nth3[AQ - 1] = th3d[clk(k + 3 + AQ)];
+      nta3[AQ - 1] = theta[clk(k + 3 + AQ)];
     } else {
       r4 = raw_at(k + 4);
       th3 = (k + 3 <= K) ? th3d_eff(k + 3) : 0.0;

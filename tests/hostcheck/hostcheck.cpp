@@ -103,7 +103,7 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
   for (int j = 1; j <= jj; j++)
     for (int i = 1; i <= ii; i++)
       if (V.ip[ij_of(V, i, j)]) {
-        if (g_uv_recompute) regrid_stream<true>(P, V, W, i, j, n);
+        if (g_uv_recompute) regrid_stream<1>(P, V, W, i, j, n);
         else regrid_stream(P, V, W, i, j, n);
       }
   int64_t unsafe = 0;
