@@ -841,8 +841,7 @@ int hycom_hybgen_set_kernel_path(int path) {
 
 int hycom_hybgen_tune(int knob, int value) {
   if (knob == 0) {
-    if (value != 3 && value != 20 && value != 21 && value != 22)
-      return fail(HYB_EINVAL, "remap variant %d: 3, 20, 21 or 22", value);
+    if (value != 3) return fail(HYB_EINVAL, "remap variant %d: only 3 is built", value);
     g_tune.remap_variant = value;
     return 0;
   }

@@ -1675,24 +1675,6 @@ HYB_HDI StagedInterfaces stage_interfaces(double* row, int ld, const double* pg,
   stage_column(row, ld, pg + S, S, K - 1);
   return StagedInterfaces{row, ld, K + 1, top, bot};
 }
-// The same staged column read one request ahead: a walk asks for prsf(1),
-// then prsf(2), prsf(3), ... each once (a new walk starts again at 1), so
-// the LDS read of level k+1 is issued when level k is handed out and has a
-// whole output's work to arrive.
-struct StagedAhead {
-  StagedInterfaces s;
-  int k = 0;
-  double cur = 0.0, nxt = 0.0;
-  HYB_HD explicit StagedAhead(const StagedInterfaces& si) : s(si) {}
-  HYB_HD double operator()(int kq) {
-    if (kq != k) {
-      cur = (kq == k + 1) ? nxt : s(kq);
-      k = kq;
-      nxt = s(kq + 1);
-    }
-    return cur;
-  }
-};
 
 // ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
