@@ -314,6 +314,57 @@ def trc_leg(lib, dd, P, sea, args, stream):
     return out
 
 
+def copy_leg(args):
+    """Copy mode (hycom_hybgen, host arrays in and out, the path the stock
+    model's call takes) on a window of the GLBb field: ms per call with the
+    PCIe transfers, the bytes each way, and the same window's device-only
+    call for comparison.  The first call fills the library's caches (the
+    read-only fields, the land values of p, pu, pv) and is not timed."""
+    import torch
+    lib = abi.load()
+    P, w = abi.workload_params(args.workload, idm=args.copy_idm, jdm=args.copy_jdm,
+                               hybmap=args.hybmap)
+    g, _ = abi.workload_params(args.workload)
+    i0, j0 = (g.idm - P.idm) // 2, (g.jdm - P.jdm) // 2
+    S = abi.make_synth(P, w, seed=args.seed, i0=i0, j0=j0, itdm=g.idm, jtdm=g.jdm)
+    print(f"[bench] copy mode: host window {P.idm}x{P.jdm}x{P.kdm}", file=sys.stderr, flush=True)
+    import numpy as np
+    h = abi.HostDomain(P).synth(S)
+    sea = int(np.count_nonzero(h.a["ip"][P.nbdy:P.nbdy + P.ii, P.nbdy:P.nbdy + P.jj]))
+    dd = abi.DeviceDomain(P, skip=("dpo",)).upload(h)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        dd.hybgen(m=1, n=2, stream=stream)
+    torch.cuda.synchronize()
+    td = []
+    for _ in range(5):
+        ts = time.perf_counter()
+        dd.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        td.append(time.perf_counter() - ts)
+    del dd
+    torch.cuda.empty_cache()
+    h.hybgen(m=1, n=2)  # fills the caches
+    tc = []
+    for q in range(3):
+        ts = time.perf_counter()
+        h.hybgen(m=1 + q % 2, n=2 - q % 2)
+        tc.append(time.perf_counter() - ts)
+    up, down = ctypes.c_int64(), ctypes.c_int64()
+    abi.check(lib.hycom_hybgen_copy_stats(ctypes.byref(up), ctypes.byref(down)))
+    tc.sort()
+    td.sort()
+    ms = 1e3 * tc[1]
+    return {"window": f"{P.idm}x{P.jdm}x{P.kdm} centre window of the GLBb field, "
+                      f"{P.ntracr} tracers, host arrays (pageable)",
+            "sea_columns": sea, "ms_per_call": ms, "device_only_ms_per_call": 1e3 * td[2],
+            "column_layers_per_s": sea * P.kdm / (ms * 1e-3),
+            "bytes_up": up.value, "bytes_down": down.value,
+            "pcie_GBs": (up.value + down.value) / (ms * 1e-3) / 1e9,
+            "about": "median of 3 calls after one that fills the caches; levels n of the "
+                     "fields the call touches go both ways, p/pu/pv come back"}
+
+
 def batch_leg(args):
     """64 regional domains of random shape (idm 100-1200, jdm 100-900, kdm
     22/32/41, shelf fraction 0-60%), FP64 PPM, stepped together by
@@ -488,6 +539,9 @@ def main():
     ap.add_argument("--batch-baseline", action="store_true",
                     help="also time the batch's domains one hycom_hybgen_device call each")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop leg")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-mode leg")
+    ap.add_argument("--copy-idm", type=int, default=1500)
+    ap.add_argument("--copy-jdm", type=int, default=1000)
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-domains", type=int, default=64)
@@ -624,10 +678,12 @@ def main():
         stokes = stokes_leg(lib, dd, P, sea, args, stream)
     if not args.no_trc:
         trc = trc_leg(lib, dd, P, sea, args, stream)
-    batch = serve = None
-    if not (args.no_batch and args.no_serve):
+    batch = serve = copy = None
+    if not (args.no_batch and args.no_serve and args.no_copy):
         del dd  # the GLBb fields make room for the other legs
         torch.cuda.empty_cache()
+    if not args.no_copy:
+        copy = copy_leg(args)
     if not args.no_batch:
         batch = batch_leg(args)
     if not args.no_serve:
@@ -695,6 +751,7 @@ def main():
         "trc": trc,
         "batch": batch,
         "serve": serve,
+        "copy_mode": copy,
     }
     if mixed:
         mixed["speedup_vs_fp64"] = mixed["value"] / value

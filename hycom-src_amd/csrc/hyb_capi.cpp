@@ -179,11 +179,18 @@ struct Mirror {
   double *ubavg = nullptr, *vbavg = nullptr;
   int32_t *ip = nullptr, *iu = nullptr, *iv = nullptr;
   hipStream_t stream = nullptr;
+  // host arrays the cached device copies came from: the fields the call
+  // only reads and never changes (theta, depths, topiso, depthu, depthv,
+  // the masks), and p, pu, pv, whose land and halo values the call keeps
+  const void* host_const[8] = {};
+  const void* host_p[3] = {};
+  int64_t up = 0, down = 0;  // bytes moved by the last call
   void release() {
     void* ptrs[] = {dp, dpo, dpu, dpv, temp, saln, th3d, u, v, tracer, p, pu, pv, theta,
                     depths, topiso, depthu, depthv, ubavg, vbavg, ip, iu, iv};
     for (void* x : ptrs)
       if (x) (void)hipFree(x);
+    if (stream) (void)hipStreamDestroy(stream);
     Shape z;
     *this = Mirror();
     shape = z;
@@ -893,16 +900,27 @@ int hycom_hybgen_timing_read(double* ms_sum, int32_t* count, int nphase) {
   return 0;
 }
 
+// Copy mode moves what the call reads and writes (hybgen.F90:14-306 with
+// the fields of hybgenaj/bj/cj): time level n of dp, dpu, dpv, temp, saln,
+// th3d, u, v and the tracers both ways, level n of ubavg/vbavg up, and p,
+// pu, pv down; with hybgen_raflag also level m of those fields and dpo.
+// theta, depths, topiso, depthu, depthv and the masks go up once per host
+// array (hycom_hybgen_copy_invalidate() drops the cache).  p, pu and pv are
+// rewritten by the call at every sea point; their land and halo values go
+// up once per host array and come back unchanged with every download.
 int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag) {
   int rc = hycom_hybgen_check_params(P);
   if (rc) return rc;
   if ((rc = check_fields(P, F, raflag))) return rc;
+  if (m < 1 || m > 2 || n < 1 || n > 2 || (raflag && m == n))
+    return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
   if ((rc = device_ready())) return rc;
   std::lock_guard<std::mutex> lk(g_mirror_mu);
   Mirror& M = g_mirror;
   Shape s = shape_of(P);
   const size_t n2 = (size_t)s.NIJ, n3 = n2 * s.K, n3p = n2 * (s.K + 1), n4 = 2 * n3;
-  const size_t n5 = n4 * (size_t)(s.ntr > 0 ? s.ntr : 1);
+  const int ntr = s.ntr > 0 ? s.ntr : 1;
+  const size_t n5 = n4 * (size_t)ntr;
   if (!(M.shape == s) || !M.dp) {
     M.release();
     HIPCHK(hipStreamCreate(&M.stream));
@@ -923,58 +941,96 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
     M.shape = s;
   }
   hipStream_t st = M.stream;
+  int64_t up = 0, down = 0;
   auto h2d = [&](void* d, const void* h, size_t bytes) {
-    return h ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+    up += (int64_t)bytes;
+    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st);
   };
   auto d2h = [&](void* h, const void* d, size_t bytes) {
-    return h ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st) : hipSuccess;
+    down += (int64_t)bytes;
+    return hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st);
   };
-  const size_t b4 = n4 * sizeof(double);
-  HIPCHK(h2d(M.dp, F->dp, b4));
-  HIPCHK(h2d(M.dpo, F->dpo, b4));
-  HIPCHK(h2d(M.dpu, F->dpu, b4));
-  HIPCHK(h2d(M.dpv, F->dpv, b4));
-  HIPCHK(h2d(M.temp, F->temp, b4));
-  HIPCHK(h2d(M.saln, F->saln, b4));
-  HIPCHK(h2d(M.th3d, F->th3d, b4));
-  HIPCHK(h2d(M.u, F->u, b4));
-  HIPCHK(h2d(M.v, F->v, b4));
-  if (P->ntracr > 0) HIPCHK(h2d(M.tracer, F->tracer, n5 * sizeof(double)));
-  HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
-  HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
-  HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
-  HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
-  HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
-  HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
-  HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
-  HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
-  HIPCHK(h2d(M.ubavg, F->ubavg, 3 * n2 * sizeof(double)));
-  HIPCHK(h2d(M.vbavg, F->vbavg, 3 * n2 * sizeof(double)));
-  HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
-  HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
-  HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
+  // the fields of time level l (1-based) that the call reads and writes
+  const size_t b3 = n3 * sizeof(double);
+  auto level = [&](int l, bool to_device) -> hipError_t {
+    const size_t o = n3 * (size_t)(l - 1);
+    double* dev[] = {M.dp, M.dpu, M.dpv, M.temp, M.saln, M.th3d, M.u, M.v};
+    double* host[] = {F->dp, F->dpu, F->dpv, F->temp, F->saln, F->th3d, F->u, F->v};
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < 8 && e == hipSuccess; q++)
+      e = to_device ? h2d(dev[q] + o, host[q] + o, b3) : d2h(host[q] + o, dev[q] + o, b3);
+    for (int t = 0; t < P->ntracr && e == hipSuccess; t++) {
+      const size_t ot = n4 * (size_t)t + o;
+      e = to_device ? h2d(M.tracer + ot, F->tracer + ot, b3)
+                    : d2h(F->tracer + ot, M.tracer + ot, b3);
+    }
+    if (to_device && e == hipSuccess) {
+      const size_t ob = n2 * (size_t)(l - 1);
+      e = h2d(M.ubavg + ob, F->ubavg + ob, n2 * sizeof(double));
+      if (e == hipSuccess) e = h2d(M.vbavg + ob, F->vbavg + ob, n2 * sizeof(double));
+    }
+    return e;
+  };
+  // read-only fields, once per host array
+  const void* hc[8] = {F->theta, F->depths, F->topiso, F->depthu, F->depthv, F->ip, F->iu, F->iv};
+  if (memcmp(hc, M.host_const, sizeof hc) != 0) {
+    HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
+    HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
+    HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
+    HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
+    HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
+    HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
+    HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
+    HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
+    memcpy(M.host_const, hc, sizeof hc);
+  }
+  // p, pu, pv: land and halo values, once per host array
+  const void* hp[3] = {F->p, F->pu, F->pv};
+  if (memcmp(hp, M.host_p, sizeof hp) != 0) {
+    HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
+    HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
+    HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
+    memcpy(M.host_p, hp, sizeof hp);
+  }
+  HIPCHK(level(n, true));
+  if (raflag) {
+    HIPCHK(level(m, true));
+    HIPCHK(h2d(M.dpo, F->dpo, n4 * sizeof(double)));
+  }
   hyb_fields D;
-  D.dp = M.dp; D.dpo = F->dpo ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
+  D.dp = M.dp; D.dpo = raflag ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
   D.temp = M.temp; D.saln = M.saln; D.th3d = M.th3d; D.u = M.u; D.v = M.v;
   D.tracer = P->ntracr > 0 ? M.tracer : nullptr;
   D.p = M.p; D.pu = M.pu; D.pv = M.pv; D.theta = M.theta;
   D.depths = M.depths; D.topiso = M.topiso; D.depthu = M.depthu; D.depthv = M.depthv;
   D.ubavg = M.ubavg; D.vbavg = M.vbavg; D.ip = M.ip; D.iu = M.iu; D.iv = M.iv;
   rc = hycom_hybgen_device(P, &D, m, n, raflag, st);
-  if (rc) return rc;
-  HIPCHK(d2h(F->dp, M.dp, b4));
-  HIPCHK(d2h(F->dpu, M.dpu, b4));
-  HIPCHK(d2h(F->dpv, M.dpv, b4));
-  HIPCHK(d2h(F->temp, M.temp, b4));
-  HIPCHK(d2h(F->saln, M.saln, b4));
-  HIPCHK(d2h(F->th3d, M.th3d, b4));
-  HIPCHK(d2h(F->u, M.u, b4));
-  HIPCHK(d2h(F->v, M.v, b4));
-  if (P->ntracr > 0) HIPCHK(d2h(F->tracer, M.tracer, n5 * sizeof(double)));
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
+  HIPCHK(level(n, false));
+  if (raflag) HIPCHK(level(m, false));
   HIPCHK(d2h(F->p, M.p, n3p * sizeof(double)));
   HIPCHK(d2h(F->pu, M.pu, n3p * sizeof(double)));
   HIPCHK(d2h(F->pv, M.pv, n3p * sizeof(double)));
   HIPCHK(hipStreamSynchronize(st));
+  M.up = up;
+  M.down = down;
+  return 0;
+}
+
+int hycom_hybgen_copy_invalidate(void) {
+  std::lock_guard<std::mutex> lk(g_mirror_mu);
+  memset(g_mirror.host_const, 0, sizeof g_mirror.host_const);
+  memset(g_mirror.host_p, 0, sizeof g_mirror.host_p);
+  return 0;
+}
+
+int hycom_hybgen_copy_stats(int64_t* bytes_up, int64_t* bytes_down) {
+  std::lock_guard<std::mutex> lk(g_mirror_mu);
+  if (bytes_up) *bytes_up = g_mirror.up;
+  if (bytes_down) *bytes_down = g_mirror.down;
   return 0;
 }
 

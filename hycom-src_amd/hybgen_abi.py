@@ -126,6 +126,8 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int]),
     "hycom_hybgen_device": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "hycom_hybgen_copy_invalidate": (ctypes.c_int, []),
+    "hycom_hybgen_copy_stats": (ctypes.c_int, [_P(ctypes.c_int64), _P(ctypes.c_int64)]),
     "hycom_hybgen_device_noalloc": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "hycom_hybgen_noalloc_capacity": (ctypes.c_size_t, []),

@@ -165,6 +165,19 @@ int hycom_hybgen_derive_vgrid(hyb_params* P,
 int hycom_hybgen(const hyb_params* P, const hyb_fields* F,
                  int m, int n, int raflag);
 
+/* Copy mode moves only what the call reads and writes: time level n of
+ * dp, dpu, dpv, temp, saln, th3d, u, v and the tracers both ways (and
+ * level m plus dpo with hybgen_raflag), level n of ubavg/vbavg up, p, pu,
+ * pv down.  theta, depths, topiso, depthu, depthv and the masks are
+ * uploaded once per host array and taken as unchanged afterwards; so are
+ * the land and halo values of p, pu, pv, which the call does not write.
+ * A caller that changes any of those in place between calls calls
+ * hycom_hybgen_copy_invalidate() first.  hycom_hybgen_copy_stats() gives
+ * the bytes the last copy-mode call moved each way.  No reference
+ * counterpart (the reference has no device copies). */
+int hycom_hybgen_copy_invalidate(void);
+int hycom_hybgen_copy_stats(int64_t* bytes_up, int64_t* bytes_down);
+
 /* Same call on device-resident fields (device pointers in F), ordered on
  * the given HIP stream (hipStream_t passed as void*, NULL = default
  * stream).  Returns after enqueueing; the caller synchronises. */

@@ -105,6 +105,60 @@ def test_copy_mode_matches_oracle(gpu):
     _assert_same(want, got, "copy mode")
 
 
+def _copy_bytes(P, raflag):
+    """Bytes copy mode moves per call once its caches hold (see
+    hycom_hybgen_copy_stats): levels n (and m, dpo with raflag)."""
+    NIJ = (P.idm + 2 * P.nbdy) * (P.jdm + 2 * P.nbdy)
+    lev = 8 * NIJ * P.kdm
+    nl = 2 if raflag else 1
+    up = nl * ((8 + P.ntracr) * lev + 2 * 8 * NIJ) + (2 * lev if raflag else 0)
+    down = nl * (8 + P.ntracr) * lev + 3 * 8 * NIJ * (P.kdm + 1)
+    return up, down
+
+
+def test_copy_mode_sequence_matches_oracle(gpu):
+    """Successive copy-mode calls on one host domain, as a model makes
+    them: time levels alternating, a Robert-Asselin call, a second domain
+    of the same shape (new host arrays), and a call after theta changed in
+    place with the cache dropped.  Every call against the oracle on the same
+    sequence, and the bytes each call moved."""
+    lib = abi.load()
+    P, w = _case("copy-sequence", idm=90, jdm=57, kdm=41, hybmap=2, ntracr=3,
+                 trcflg=[2, 0, 0])
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=31))
+    rng = np.random.default_rng(31)
+    d0.a["dpo"][...] = np.abs(d0.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d0.a["dp"].shape)))
+    # land p, pu, pv hold values the call must keep
+    for name in ("p", "pu", "pv"):
+        d0.a[name][...] = rng.standard_normal(d0.a[name].shape)
+    want, got = d0.copy(), d0.copy()
+    up = ctypes.c_int64()
+    down = ctypes.c_int64()
+    for q, (m, n, ra) in enumerate([(1, 2, 0), (2, 1, 0), (1, 2, 1), (2, 1, 0)]):
+        print(f"[call] {q} m={m} n={n} raflag={ra}", flush=True)
+        ol.run_oracle(want, m=m, n=n, raflag=ra, threads=8)
+        got.hybgen(m=m, n=n, raflag=ra)
+        _assert_same(want, got, f"copy mode call {q}")
+        abi.check(lib.hycom_hybgen_copy_stats(ctypes.byref(up), ctypes.byref(down)))
+        if q >= 1:  # caches filled by the first call
+            assert (up.value, down.value) == _copy_bytes(P, ra), (q, up.value, down.value)
+    # a second domain of the same shape: its own land values of p, pu, pv
+    d1 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=32))
+    for name in ("p", "pu", "pv"):
+        d1.a[name][...] = -rng.random(d1.a[name].shape)
+    w1, g1 = d1.copy(), d1.copy()
+    ol.run_oracle(w1, threads=8)
+    g1.hybgen(m=1, n=2)
+    _assert_same(w1, g1, "copy mode, second domain")
+    # theta changed in place: the caller drops the cache
+    want.a["theta"][...] += 0.01
+    got.a["theta"][...] += 0.01
+    abi.check(lib.hycom_hybgen_copy_invalidate())
+    ol.run_oracle(want, m=1, n=2, threads=8)
+    got.hybgen(m=1, n=2)
+    _assert_same(want, got, "copy mode after theta changed")
+
+
 def test_device_synth_matches_host_synth(gpu):
     P, w = _case("synth", idm=130, jdm=77, kdm=41, ntracr=4, trcflg=[2, 0, 0, 0])
     S = abi.make_synth(P, w, seed=21)
