@@ -540,6 +540,8 @@ def main():
                     help="also time the batch's domains one hycom_hybgen_device call each")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop leg")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-mode leg")
+    ap.add_argument("--no-denfix", action="store_true", help="skip the HYBGEN_DENFIX leg")
+    ap.add_argument("--no-raflag", action="store_true", help="skip the Robert-Asselin leg")
     ap.add_argument("--copy-idm", type=int, default=1500)
     ap.add_argument("--copy-jdm", type=int, default=1000)
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second")
@@ -670,6 +672,52 @@ def main():
                 "ms_p50": 1e3 * tw[len(tw) // 2],
                 "ms_p99": 1e3 * tw[min(len(tw) - 1, int(round(0.99 * (len(tw) - 1))))]}
         print(f"[bench] weno {weno['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    denfix = None
+    if not args.no_denfix:
+        # the same field and steps with HYBGEN_DENFIX (the regional build's
+        # fixed-layer search, hybgen.F90:496-549)
+        P.ldenfix = 1
+        for _ in range(args.warmup):
+            dd.hybgen(m=1, n=2, stream=stream)
+        torch.cuda.synchronize()
+        tw = []
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            dd.hybgen(m=1, n=2, stream=stream)
+            torch.cuda.synchronize()
+            tw.append(time.perf_counter() - ts)
+        P.ldenfix = 0
+        tw.sort()
+        denfix = {"value": sea * P.kdm * len(tw) / sum(tw), "unit": "column-layers/s",
+                  "ms_per_step": 1e3 * sum(tw) / len(tw),
+                  "ms_p50": 1e3 * tw[len(tw) // 2]}
+        print(f"[bench] denfix {denfix['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    raflag = None
+    if not args.no_raflag:
+        # hybgen(m,n,.true.): the Robert-Asselin branch (hybgencj) and the
+        # second u/v pass on level m, on freshly generated input (both time
+        # levels and dpo from the generator), generator outside the events
+        import torch as _t
+        dd.t["dpo"] = _t.zeros_like(dd.t["dp"])
+        tr = []
+        for q in range(4):
+            dd.synth(S, stream=stream)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            dd.hybgen(m=1, n=2, raflag=1, stream=stream)
+            e1.record()
+            torch.cuda.synchronize()
+            if q:
+                tr.append(e0.elapsed_time(e1))
+        del dd.t["dpo"]
+        torch.cuda.empty_cache()
+        tr.sort()
+        raflag = {"ms_per_call": tr[1], "all_ms": tr,
+                  "value": sea * P.kdm / (tr[1] * 1e-3), "unit": "column-layers/s",
+                  "about": "hybgen(1,2,.true.) on freshly generated input, median of 3 "
+                           "after one warm-up call"}
+        print(f"[bench] raflag {tr[1]:.2f} ms/call", file=sys.stderr, flush=True)
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(lib, dd, S, sea, args, stream)
@@ -746,6 +794,8 @@ def main():
         "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
         "cpu_baseline": cpu,
         "weno": weno,
+        "denfix": denfix,
+        "raflag": raflag,
         "mixed": mixed,
         "stokes": stokes,
         "trc": trc,

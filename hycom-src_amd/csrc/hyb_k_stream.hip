@@ -102,7 +102,7 @@ static void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, 
   launch_uv_fallback(P, V, W, nl, s);
 }
 
-bool stream_path_ok(const KParams& P) { return !P.ldenfix && P.nhybrd == P.kk; }
+bool stream_path_ok(const KParams& P) { return P.nhybrd == P.kk; }
 
 int uv_method_of(const KParams& P) { return uv_method(P); }
 

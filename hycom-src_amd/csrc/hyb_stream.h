@@ -55,6 +55,10 @@ HYB_HD double dp0ij_qh(const KParams& P, double d, double pk, double dp0cum_k, d
   *qh = 1.0 / (1.0 + qts * (P.hybrlx - 1.0));
   return dmin(q, d);
 }
+// dp0ij(k) when qdep < 1 (hybgen.F90:405-418)
+HYB_HD double dp0ij_tf_of(const KParams& P, double qdep, int k) {
+  return qdep * P.dp0k[k - 1] + (1.0 - qdep) * P.ds0k[k - 1];
+}
 HYB_HD double dp0ij_only(const KParams& P, double d, double pk, double dp0cum_k) {
   double q;
   if (d <= P.dp00i) {
@@ -123,6 +127,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   // overturn) and every new one to be 0 or in [2^-60, 2^60], the old ones
   // in order; a column where that fails goes to the fallback remap.
   bool tame = (FK(dpc, 1) >= 0.0) & tame_p(pk1);
+  bool mono = FK(dpc, 1) >= 0.0;  // p(1..kk+1) in order
   int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
   int kp = 2;
   auto level = [&](int k, double dk) {
@@ -130,6 +135,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     pk1 = pk + dk;
     pres[(int64_t)k * S] = pk1;
     tame = tame & (dk >= 0.0) & tame_p(pk1);
+    mono = mono & (dk >= 0.0);
     if (k >= 3 && pk1 - pk >= dpthin) kp = k;  // 590-598: the deepest such k
     double dtf = 0.0, cumk1;
     if (tf) {
@@ -147,7 +153,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
         qfix = k + 1;
       }
     }
-    if (phase == 2) {  // 550-573
+    if (phase == 2 && !P.ldenfix) {  // 550-573
       const double d0k = tf ? dtf : dp0ij_only(P, P.dp0k[k - 1], pk, cumk);
       if (pk1 > cumk1 + 0.1 * d0k) {
         if (fixlay > fixall) {
@@ -182,6 +188,76 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     }
   }
   const double zx = pk1;  // p(kk+1); the overturn does not move it
+
+  if (P.ldenfix) {
+    // HYBGEN_DENFIX, 496-549: the fixed layers end where the target
+    // interface dp0cum(k+1) is deeper than p(k+1) by a tenth of dp0ij(k)
+    // and theta(k) is denser than th3d interpolated to dp0cum(k+1).  The
+    // search for the layer l holding dp0cum(k+1) moves down from the last
+    // one found (lf), over the interfaces pass 1 has just stored, before the
+    // overturn moves any of them.
+    auto pp = [&](int x) -> double { return pres[(int64_t)(x - 1) * S]; };  // p(x)
+    const int fixall_d = fixlay;
+    int lf = 1;
+    double plf = pp(1), plf1 = pp(2);  // p(lf), p(lf+1)
+    // With the interfaces in order, no l >= lf has p(l) <= zk <= p(l+1)
+    // when zk lies above p(lf) or below the bottom p(kk+1), so that search
+    // is skipped (it would find nothing); out of order, every search runs
+    // as written.
+    // dp0cum(k) and dp0cum(k+1) at k = 2, then advanced with the same
+    // additions as 399-477 when qdep < 1
+    double c_k = tf ? 0.0 + dp0ij1 : P.dp0cum[1];
+    for (int k = 2; k <= fixall_d; k++) c_k = tf ? c_k + dp0ij_tf_of(P, qdep, k) : P.dp0cum[k];
+    for (int k = fixall_d + 1; k <= K; k++) {
+      const double c_k1 = tf ? c_k + dp0ij_tf_of(P, qdep, k) : P.dp0cum[k];  // dp0cum(k+1)
+      const double pk_d = pp(k), pk1_d = pp(k + 1), thk_d = FK(theta, k);
+      double denfix = FK(th3d, K);
+      const double zk = c_k1;
+      if (!(mono && (zk > zx || zk < plf))) {
+        double pl = plf, pl1 = plf1;
+        for (int l = lf; l <= K; l++) {
+          if (l > lf) {
+            pl = pl1;
+            pl1 = pp(l + 1);
+          }
+          if (pl <= zk && pl1 >= zk) {
+            const double z0 = 0.5 * (pl + pl1);
+            if (zk <= z0) {
+              if (l == 1) {
+                denfix = FK(th3d, 1);
+              } else {
+                const double zm = 0.5 * (pp(l - 1) + pl);
+                const double q = (z0 - zk) / (z0 - zm);
+                denfix = q * FK(th3d, l - 1) + (1.0 - q) * FK(th3d, l);
+              }
+            } else {
+              if (pl1 == zx) {  // p(l+1) == p(nhybrd+1), nhybrd == kk
+                denfix = FK(th3d, K);
+              } else {
+                const double zp = 0.5 * (pl1 + pp(l + 2));
+                const double q = (zk - z0) / (zp - z0);
+                denfix = q * FK(th3d, l + 1) + (1.0 - q) * FK(th3d, l);
+              }
+            }
+            lf = l;
+            plf = pl;
+            plf1 = pl1;
+            break;
+          }
+        }
+      }
+      const double d0k = tf ? dp0ij_tf_of(P, qdep, k) : dp0ij_only(P, P.dp0k[k - 1], pk_d, c_k);
+      if (pk1_d > c_k1 + 0.1 * d0k && thk_d > denfix) {
+        if (fixlay > fixall_d) {
+          if (pk_d > c_k) fixlay = fixlay - 1;
+        }
+        break;
+      }
+      if (k > qfix) qfix = k;  // qhrlx(k) = 1
+      fixlay = fixlay + 1;
+      c_k = c_k1;
+    }
+  }
 
   // ------------------------------------------------ pass 2: 606-970
   // The overturn moves p(kp0) or p(kp0-1) only; dp0ij and qhrlx are

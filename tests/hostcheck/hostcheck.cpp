@@ -211,7 +211,7 @@ extern "C" int hostcheck_hybgen_stream(const hyb_params* P, const hyb_fields* F,
                                        int raflag) {
   KParams K = kparams_of(P);
   View V = view_of(P, F);
-  if (K.ldenfix || K.nhybrd != K.kk) return -2;
+  if (K.nhybrd != K.kk) return -2;
   g_uv_fallback = 0;
   run_stream(K, V, m, n, raflag);
   return 0;

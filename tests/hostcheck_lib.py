@@ -34,7 +34,7 @@ def lib():
 
 
 def covers(family, P):
-    ok = P.ldenfix == 0 and P.nhybrd == P.kdm
+    ok = P.nhybrd == P.kdm
     if family in ("streaming", "streaming-recompute"):
         return ok
     return True

@@ -18,14 +18,14 @@ FAMILIES = {1: "lane-arrays", 3: "streaming"}
 @pytest.fixture(params=[1, 3], ids=list(FAMILIES.values()))
 def kernel_path(request, gpu):
     """Runs the test once per kernel family.  A family that does not cover
-    the case's parameters (streaming: nhybrd==kdm, no HYBGEN_DENFIX) is
+    the case's parameters (streaming: nhybrd==kdm) is
     replaced by the lane-array kernels."""
     yield request.param
     abi.check(abi.load().hycom_hybgen_set_kernel_path(0))
 
 
 def _select(family, P):
-    ok = P.ldenfix == 0 and P.nhybrd == P.kdm
+    ok = P.nhybrd == P.kdm
     if family == 3 and not ok:
         family = 1
     print(f"[kernels] {FAMILIES[family]}", flush=True)
