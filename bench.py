@@ -700,7 +700,11 @@ def main():
         import torch as _t
         dd.t["dpo"] = _t.zeros_like(dd.t["dp"])
         tr = []
+        rms = (ctypes.c_double * np_)()
+        rcnt = (ctypes.c_int32 * np_)()
         for q in range(4):
+            if q == 1:
+                abi.check(lib.hycom_hybgen_timing(1))
             dd.synth(S, stream=stream)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -710,10 +714,13 @@ def main():
             torch.cuda.synchronize()
             if q:
                 tr.append(e0.elapsed_time(e1))
+        abi.check(lib.hycom_hybgen_timing_read(rms, rcnt, np_))
+        abi.check(lib.hycom_hybgen_timing(0))
         del dd.t["dpo"]
         torch.cuda.empty_cache()
         tr.sort()
         raflag = {"ms_per_call": tr[1], "all_ms": tr,
+                  "phase_ms": {PHASES[q]: rms[q] / rcnt[q] for q in range(np_) if rcnt[q]},
                   "value": sea * P.kdm / (tr[1] * 1e-3), "unit": "column-layers/s",
                   "about": "hybgen(1,2,.true.) on freshly generated input, median of 3 "
                            "after one warm-up call"}

@@ -225,8 +225,9 @@ PhaseTimer* timer_take() {
   return t;
 }
 
-// Workspace of the streaming kernels (pres, kp, lcm, uvf), one per
-// domain, keyed by the domain's p array; allocated on first use.
+// Workspace of the streaming kernels (pres, kp, lcm, uvf, and ras for the
+// Robert-Asselin branch, allocated with the first hybgen_raflag call), one
+// per domain, keyed by the domain's p array; allocated on first use.
 struct Workspace {
   const void* key = nullptr;
   int64_t nij = 0;
@@ -235,17 +236,19 @@ struct Workspace {
   int32_t* kp = nullptr;
   uint64_t* lcm = nullptr;
   int32_t* uvf = nullptr;
+  double* ras = nullptr;
 };
 std::mutex g_ws_mu;
 std::vector<Workspace> g_ws;
 int g_kernel_path = 0;  // 0 auto, 1 per-lane arrays, 3 streaming (2 was register-resident)
 StreamTuning g_tune;
 
-int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
+int workspace_for(const void* key, int64_t nij, int K, Scratch* W, bool ra = false) {
   std::lock_guard<std::mutex> lk(g_ws_mu);
   for (auto& w : g_ws) {
     if (w.key == key && w.nij == nij && w.K == K) {
-      *W = Scratch{w.pres, w.kp, w.lcm, w.uvf};
+      if (ra && !w.ras) HIPCHK(hipMalloc(&w.ras, sizeof(double) * 3 * nij * K));
+      *W = Scratch{w.pres, w.kp, w.lcm, w.uvf, w.ras};
       return 0;
     }
   }
@@ -255,6 +258,7 @@ int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
       (void)hipFree(it->kp);
       (void)hipFree(it->lcm);
       (void)hipFree(it->uvf);
+      if (it->ras) (void)hipFree(it->ras);
       g_ws.erase(it);
       break;
     }
@@ -267,8 +271,9 @@ int workspace_for(const void* key, int64_t nij, int K, Scratch* W) {
   HIPCHK(hipMalloc(&w.kp, sizeof(int32_t) * nij));
   HIPCHK(hipMalloc(&w.lcm, sizeof(uint64_t) * nij));
   HIPCHK(hipMalloc(&w.uvf, sizeof(int32_t) * 2 * nij));
+  if (ra) HIPCHK(hipMalloc(&w.ras, sizeof(double) * 3 * nij * K));
   g_ws.push_back(w);
-  *W = Scratch{w.pres, w.kp, w.lcm, w.uvf};
+  *W = Scratch{w.pres, w.kp, w.lcm, w.uvf, w.ras};
   return 0;
 }
 
@@ -712,7 +717,7 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
   if (use_stream) {
     Scratch W;
-    if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W))) return rc;
+    if ((rc = workspace_for(F->p, V.NIJ, P->kdm, &W, raflag != 0))) return rc;
     HIPCHK(launch_hybgen_stream(K, V, W, m, n, raflag, g_tune, (hipStream_t)stream,
                                 timer_take()));
   } else {
@@ -737,13 +742,14 @@ void* g_noalloc_ptr[64] = {};
 
 size_t noalloc_round(size_t b) { return (b + 255) & ~size_t(255); }
 
-// bytes of pres, kp, lcm and uvf for NIJ points and K layers, each piece
-// 256-byte aligned
-size_t noalloc_need(int64_t nij, int K) {
+// bytes of pres, kp, lcm and uvf for NIJ points and K layers (and ras
+// with hybgen_raflag), each piece 256-byte aligned
+size_t noalloc_need(int64_t nij, int K, bool ra = false) {
   return noalloc_round(sizeof(double) * (size_t)nij * (size_t)(K + 1)) +
          noalloc_round(sizeof(int32_t) * (size_t)nij) +
          noalloc_round(sizeof(uint64_t) * (size_t)nij) +
-         noalloc_round(sizeof(int32_t) * 2 * (size_t)nij);
+         noalloc_round(sizeof(int32_t) * 2 * (size_t)nij) +
+         (ra ? noalloc_round(sizeof(double) * 3 * (size_t)nij * (size_t)K) : 0);
 }
 
 // the current device's copy of scratch, looked up once per device
@@ -796,7 +802,7 @@ int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m,
     return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
   const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
   const int64_t nij = shape_of(P).NIJ;
-  const size_t need = noalloc_need(nij, P->kdm);
+  const size_t need = noalloc_need(nij, P->kdm, raflag != 0);
   if (use_stream && need > kNoallocBytes)
     return fail(HYB_EINVAL,
                 "domain of %lld points x %d layers needs %zu bytes of workspace; the fixed "
@@ -820,6 +826,8 @@ int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m,
   W.lcm = (uint64_t*)base;
   base += noalloc_round(sizeof(uint64_t) * (size_t)V.NIJ);
   W.uvf = (int32_t*)base;
+  base += noalloc_round(sizeof(int32_t) * 2 * (size_t)V.NIJ);
+  W.ras = raflag ? (double*)base : nullptr;
   // the u/v phase stays on the caller's stream: a second stream would be
   // created on first use
   StreamTuning tune = g_tune;

@@ -1079,6 +1079,8 @@ struct Scratch {
   int32_t* kp;        // deepest inflated layer after overturn
   uint64_t* lcm;      // PCM-layer mask after the dx0k pass
   int32_t* uvf;       // (2, NIJ) u/v points left to the copy-first kernel
+  double* ras;        // (NIJ, K, 3) outputs of the RA walks that cannot go in place
+                      // (hybgen_raflag only; NULL: such columns go to column_ra)
 };
 
 // Row of per-lane storage in LDS (device) or a local array (host).

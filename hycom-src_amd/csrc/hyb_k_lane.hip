@@ -68,6 +68,17 @@ __global__ void __launch_bounds__(kWave) k_uv_fb(KParams P, View V, Scratch W, i
   point_c<KC>(P, V, i, j, nl, isv);
 }
 
+template <int KC>
+__global__ void __launch_bounds__(kWave) k_ra_fb(KParams P, View V, Scratch W, int m, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;
+  if (i > P.ii) return;
+  const int64_t c0 = ij_of(V, i, j);
+  if (V.ip[c0] == 0) return;
+  if (!(W.kp[c0] & kUnsafe)) return;
+  column_ra<KC>(P, V, i, j, m, n, lcm_init(P));
+}
+
 #define HYB_KC_DISPATCH(KK, CALL) \
   do {                            \
     if ((KK) <= 24) {             \
@@ -99,6 +110,12 @@ void launch_remap_fallback(const KParams& P, const View& V, const Scratch& W, in
                            hipStream_t s) {
   dim3 gw2((P.ii + kWave - 1) / kWave, P.jj);
   hipLaunchKernelGGL(k_remap_fb, gw2, dim3(kWave), 0, s, P, V, W, n);
+}
+
+void launch_ra_fallback(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                        hipStream_t s) {
+  dim3 gw2((P.ii + kWave - 1) / kWave, P.jj);
+  HYB_KC_DISPATCH(P.kk, hipLaunchKernelGGL(k_ra_fb<KC>, gw2, dim3(kWave), 0, s, P, V, W, m, n));
 }
 
 void launch_uv_fallback(const KParams& P, const View& V, const Scratch& W, int nl,

@@ -3,4 +3,5 @@
 #include "hyb_stream.h"
 #define HYB_M 3
 #define HYB_LAUNCH launch_remap_stream_m3
+#define HYB_LAUNCH_RA launch_ra_stream_m3
 #include "hyb_k_remap.inc"

@@ -170,7 +170,13 @@ hipError_t launch_hybgen_stream(const KParams& P, const View& V, const Scratch& 
   if (raflag) {
     {
       PhaseMark pm(T, s, 3);
-      launch_ra_lane(P, V, m, n, s);
+      switch (P.hybmap) {
+        case 0: launch_ra_stream_m0(P, V, W, m, n, s); break;
+        case 1: launch_ra_stream_m1(P, V, W, m, n, s); break;
+        case 2: launch_ra_stream_m2(P, V, W, m, n, s); break;
+        default: launch_ra_stream_m3(P, V, W, m, n, s); break;
+      }
+      launch_ra_fallback(P, V, W, m, n, s);
     }
     {
       PhaseMark pm(T, s, 4);

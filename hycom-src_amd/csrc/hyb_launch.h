@@ -40,6 +40,9 @@ void launch_remap_fallback(const KParams& P, const View& V, const Scratch& W, in
                            hipStream_t s);
 void launch_uv_fallback(const KParams& P, const View& V, const Scratch& W, int nl,
                         hipStream_t s);
+// column_ra for the columns the streaming RA kernel flagged in W.kp
+void launch_ra_fallback(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                        hipStream_t s);
 // hyb_k_remap_m*.hip: streaming remap of the p-columns for one remapper;
 // variant selects fields per walk and the prefetch depth
 void launch_remap_stream_m0(const KParams& P, const View& V, const Scratch& W, int n,
@@ -50,5 +53,14 @@ void launch_remap_stream_m2(const KParams& P, const View& V, const Scratch& W, i
                             int variant, hipStream_t s);
 void launch_remap_stream_m3(const KParams& P, const View& V, const Scratch& W, int n,
                             int variant, hipStream_t s);
+// the Robert-Asselin branch (hybgencj) on the streaming design, per remapper
+void launch_ra_stream_m0(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                         hipStream_t s);
+void launch_ra_stream_m1(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                         hipStream_t s);
+void launch_ra_stream_m2(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                         hipStream_t s);
+void launch_ra_stream_m3(const KParams& P, const View& V, const Scratch& W, int m, int n,
+                         hipStream_t s);
 
 }  // namespace hyb

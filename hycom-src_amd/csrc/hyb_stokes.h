@@ -26,12 +26,14 @@ HYB_HD int64_t stokes_ij(const KStokes& Q, int i, int j) {
 // Old interfaces pres(1) = 0, pres(k+1) = min(sdzi(k), p(kk+1))
 // (mod_stokes.F90:675-681), for walk_core.
 struct PiStokes {
+  static constexpr bool kRawThk = false;
   const double* sdzi;
   double pbot;
   HYB_HD double first() const { return 0.0; }
   HYB_HD double raw(int x) const { return sdzi[x - 1]; }
   HYB_HD double comb(double, double r) const { return dmin(r, pbot); }
   HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
+  HYB_HD double thk(int) const { return 0.0; }
 };
 
 // New interfaces prsf(1) = 0, prsf(k+1) = min(prsf(k) + dp(k), p(kk+1))

@@ -126,7 +126,8 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
   // The remap's fast quotients need every old interface (after the
   // overturn) and every new one to be 0 or in [2^-60, 2^60], the old ones
   // in order; a column where that fails goes to the fallback remap.
-  bool tame = (FK(dpc, 1) >= 0.0) & tame_p(pk1);
+  bool tame = (FK(dpc, 1) >= 0.0);
+  tame &= tame_p(pk1);
   bool mono = FK(dpc, 1) >= 0.0;  // p(1..kk+1) in order
   int fixlay = 1, fixall = 1, phase = 1, qfix = 2;  // qhrlx(x) = 1 for x <= qfix
   int kp = 2;
@@ -134,8 +135,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     pk = pk1;
     pk1 = pk + dk;
     pres[(int64_t)k * S] = pk1;
-    tame = tame & (dk >= 0.0) & tame_p(pk1);
-    mono = mono & (dk >= 0.0);
+    tame &= (dk >= 0.0);
+    tame &= tame_p(pk1);
+    mono &= (dk >= 0.0);
     if (k >= 3 && pk1 - pk >= dpthin) kp = k;  // 590-598: the deepest such k
     double dtf = 0.0, cumk1;
     if (tf) {
@@ -365,7 +367,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
       const double p_hat = q * (pkp - pkk);
       const double pknew = pkk + p_hat;
       pres[(int64_t)(k - 1) * S] = pknew;
-      tame = tame & tame_p(pknew) & (pknew >= pkm) & (pknew <= pkp);
+      tame &= tame_p(pknew);
+      tame &= (pknew >= pkm);
+      tame &= (pknew <= pkp);
       FK(temp, k) = FK(temp, k) + (q / (1.0 - q)) * (FK(temp, k) - FK(temp, k - 1));
       FK(saln, k) = FK(saln, k) + (q / (1.0 - q)) * (FK(saln, k) - FK(saln, k - 1));
       FK(th3d, k) = sig(FK(temp, k), FK(saln, k)) - P.thbase;
@@ -483,7 +487,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
     FK(dpc, jx) = d;
     prs = prs + d;
     pg[(int64_t)jx * S] = prs;
-    tame = tame & tame_p(prs);
+    tame &= tame_p(prs);
     pf = v;
     // remap output jx ends in old layer lb <= jx-3, before the old layer
     // that reaches the bottom, only if pres(jx-2) >= zb = min(prsf(jx+1),zx)
@@ -752,22 +756,43 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
 // ---------------------------------------------------------------------
 // raw(x) is the memory word that gives pi(x+1) from pi(x) through comb(),
 // so a walk can issue it early.
+// kRawThk: the walk takes the old thicknesses dprs(x) from thk(x) as
+// stored instead of as differences of the interfaces.
 struct PiArray {  // p-columns: the overturned interfaces left by regrid_stream
+  static constexpr bool kRawThk = false;
   const double* pres;
   int64_t S;
+  HYB_HD double thk(int) const { return 0.0; }
   HYB_HD double first() const { return pres[0]; }
   HYB_HD double raw(int x) const { return pres[(int64_t)x * S]; }
   HYB_HD double comb(double, double r) const { return r; }
   HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
 };
 struct PiSum {  // u/v points: running sums of the old dpu (hybgen.F90:171-195)
+  static constexpr bool kRawThk = false;
   const double* dpu;
   int64_t S;
+  HYB_HD double thk(int) const { return 0.0; }
   HYB_HD double first() const { return 0.0; }
   HYB_HD double raw(int x) const { return dpu[(int64_t)(x - 1) * S]; }
   HYB_HD double comb(double pix, double r) const { return pix + r; }
   HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
 };
+// hybgencj (hybgen.F90:1909-2274): the old interfaces are running sums of
+// the old dp(m), kept in the workspace, and the old thicknesses are dp(m)
+// itself (hybgencj passes dprs(k) = dp(k,m), not pres(k+1)-pres(k)).
+struct PiRa {
+  static constexpr bool kRawThk = true;
+  const double* pres;  // the running sums, level 1 of the column
+  const double* dpm;   // dp(:,m), level 1 of the column
+  int64_t S;
+  HYB_HD double first() const { return pres[0]; }
+  HYB_HD double raw(int x) const { return pres[(int64_t)x * S]; }
+  HYB_HD double comb(double, double r) const { return r; }
+  HYB_HD double nextfrom(int x, double pix) const { return comb(pix, raw(x)); }
+  HYB_HD double thk(int x) const { return dpm[(int64_t)(x - 1) * S]; }
+};
+
 // New interfaces prsf(k): p-columns keep them in the lane's LDS column
 // (LaneRow); u/v points keep the new thicknesses there and sum them as
 // the walk asks for prsf(1), prsf(2), ... (hybgen.F90:1806-1810).
@@ -1234,6 +1259,10 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   CoefState<CR, NF> cst;                       // reconstruction state
   RawT sq[NF][PQ];                             // s(l+3..l+2+PD), loaded ahead
   double prq[PQ];                              // raw(l+3..l+2+PD) of pi
+  // the old thicknesses dprs(l-1..l+2) and dprs(l+3..l+2+PD) loaded ahead,
+  // when the source gives them as stored (kRawThk)
+  constexpr bool RAWT = PI::kRawThk;
+  double tv0 = 0.0, tv1 = 0.0, tv2 = 0.0, tv3 = 0.0, tq[PQ];
   int xl = 4;                                  // next level to load
   // s(x) with the massless fill (972-996) and the running min/max
   // (e.g. 2727-2731: si_min/si_max over k = 1..kk in order)
@@ -1289,6 +1318,13 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   if (K + 1 >= 4) pv4 = pi.nextfrom(3, pv3);
 #pragma unroll
   for (int q = 0; q < PD; q++) prq[q] = (4 + q <= K) ? pi.raw(4 + q) : 0.0;
+  if constexpr (RAWT) {
+    tv1 = pi.thk(1);
+    if (K >= 2) tv2 = pi.thk(2);
+    if (K >= 3) tv3 = pi.thk(3);
+#pragma unroll
+    for (int q = 0; q < PQ; q++) tq[q] = (4 + q <= K) ? pi.thk(4 + q) : 0.0;
+  }
   xl = 4 + PD;
   // layers thicker than thin, known up to l+2 (the u/v store rule)
   uint64_t thick = 0;
@@ -1299,7 +1335,8 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   cst.qdpjp_l = 0.0;
   cst.wq_l = 0.0;
   if (K >= 2) {
-    const CR d1 = gmax((CR)(pv2 - pv1), (CR)thin), d2 = gmax((CR)(pv3 - pv2), (CR)thin);
+    const CR d1 = gmax((CR)(RAWT ? tv1 : pv2 - pv1), (CR)thin),
+             d2 = gmax((CR)(RAWT ? tv2 : pv3 - pv2), (CR)thin);
     if constexpr (FD64) {
       cst.qdpjp_l = qrcp(rcp64(d1 + d2));
       cst.wq_l = cst.qdpjp_l;
@@ -1369,8 +1406,8 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
 
 #pragma unroll 1
   for (int l = 1; l <= K; l++) {
-    const double dprs_m = pv1 - pv0, dprs_0 = pv2 - pv1, dprs_p = pv3 - pv2,
-                 dprs_p2 = pv4 - pv3;
+    const double dprs_m = RAWT ? tv0 : pv1 - pv0, dprs_0 = RAWT ? tv1 : pv2 - pv1,
+                 dprs_p = RAWT ? tv2 : pv3 - pv2, dprs_p2 = RAWT ? tv3 : pv4 - pv3;
     const double dpi_m = dmax(dprs_m, thin), dpi_0 = dmax(dprs_0, thin),
                  dpi_p = dmax(dprs_p, thin), dpi_p2 = dmax(dprs_p2, thin);
     const bool lc_l = lcmb(l);
@@ -1636,6 +1673,15 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
       prq[PQ - 1] = (l + 3 + PD <= K) ? pi.raw(l + 3 + PD) : 0.0;
     }
     if (l + 3 <= K && dmax(pv4 - pv3, thin) > thin) thick |= 1ull << (l + 2);
+    if constexpr (RAWT) {
+      tv0 = tv1;
+      tv1 = tv2;
+      tv2 = tv3;
+      tv3 = (l + 3 <= K) ? tq[0] : 0.0;
+#pragma unroll
+      for (int q = 0; q < PQ - 1; q++) tq[q] = tq[q + 1];
+      tq[PQ - 1] = (l + 3 + PQ <= K) ? pi.thk(l + 3 + PQ) : 0.0;
+    }
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       s0[f] = s1[f];
@@ -1667,9 +1713,11 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
 // fields.  Every walk is instantiated for its exact field count, so no
 // field of a walk is switched off at run time.
 // ---------------------------------------------------------------------
-template <int METHOD, int NW, int PD, bool FAST, bool NT, class PO>
-HYB_HDI void remap_walk(const KParams& P, const View& V, int64_t c0, int n, int g0, int K,
-                        double zx, const double* pres, int kp, uint64_t lcm, PO& po) {
+// dst: where the outputs go (NULL: in place, over the fields read)
+template <int METHOD, int NW, int PD, bool FAST, bool NT, class PO, class PI>
+HYB_HDI void remap_walk_pi(const KParams& P, const View& V, int64_t c0, int n, int g0, int K,
+                           double zx, const PI& pi, int kp, uint64_t lcm, PO& po, int KO,
+                           double* const* dst = nullptr) {
   const int64_t S = V.NIJ;
   double* fp[NW];
 #pragma unroll
@@ -1680,9 +1728,14 @@ HYB_HDI void remap_walk(const KParams& P, const View& V, int64_t c0, int n, int 
     else fp[f] = V.tracer + c0 + levt(V, 1, n, g - 1);
   }
   double* th = (g0 == 0) ? V.th3d + c0 + lev4(V, 1, n) : nullptr;
-  PiArray pi{pres, S};
-  FieldSink<NW, NT> sink{fp, NW, th, P.thbase, S};
-  walk_core<METHOD, NW, PD, FAST, false, true>(K, zx, pi, po, sink, fp, NW, kp, lcm, S);
+  FieldSink<NW, NT> sink{dst ? dst : fp, NW, th, P.thbase, S};
+  walk_core<METHOD, NW, PD, FAST, false, true>(K, zx, pi, po, sink, fp, NW, kp, lcm, S, KO);
+}
+template <int METHOD, int NW, int PD, bool FAST, bool NT, class PO>
+HYB_HDI void remap_walk(const KParams& P, const View& V, int64_t c0, int n, int g0, int K,
+                        double zx, const double* pres, int kp, uint64_t lcm, PO& po) {
+  remap_walk_pi<METHOD, NW, PD, FAST, NT>(P, V, c0, n, g0, K, zx, PiArray{pres, V.NIJ}, kp, lcm,
+                                          po, 0);
 }
 
 // the last walk, over r < NF fields (r = 0: none)
@@ -1719,14 +1772,18 @@ HYB_HDI void remap_fields(const KParams& P, const View& V, const Scratch& W, int
 // (stride ld), 16 loads in flight before the first is stored: a column of
 // up to 65 levels waits on memory at most five times instead of once per
 // level or four.
-HYB_HDI void stage_column(double* row, int ld, const double* pg, int64_t S, int n) {
+// (ldw, when not 0, replaces ld as a 64-bit stride: a copy between two
+// columns of global memory)
+HYB_HDI void stage_column(double* row, int ld, const double* pg, int64_t S, int n,
+                          int64_t ldw = 0) {
+  const int64_t st = ldw ? ldw : (int64_t)ld;
   for (int k0 = 0; k0 < n; k0 += 16) {
     double t[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) t[q] = (k0 + q < n) ? pg[(int64_t)(k0 + q) * S] : 0.0;
 #pragma unroll
     for (int q = 0; q < 16; q++)
-      if (k0 + q < n) row[(int64_t)(k0 + q) * ld] = t[q];
+      if (k0 + q < n) row[(int64_t)(k0 + q) * st] = t[q];
   }
 }
 
@@ -1751,6 +1808,203 @@ HYB_HDI StagedInterfaces stage_interfaces(double* row, int ld, const double* pg,
   stage_column(row, ld, pg + S, S, K - 1);
   return StagedInterfaces{row, ld, K + 1, top, bot};
 }
+
+// the last RA walk, over r < NF fields (r = 0: none)
+template <int METHOD, int NW, int PD, class PO>
+HYB_HDI void ra_walk(const KParams& P, const View& V, const Scratch& W, int64_t c0, int m, int g0,
+                     int K, double zx, const PiRa& pi, uint64_t lcm, PO& po, bool staged) {
+  if (!staged) {
+    remap_walk_pi<METHOD, NW, PD, false, false>(P, V, c0, m, g0, K, zx, pi, K, lcm, po, K);
+    return;
+  }
+  // the outputs go to the workspace, then over the fields once the walk has
+  // read every level
+  const int64_t S = V.NIJ;
+  double* dst[NW];
+#pragma unroll
+  for (int f = 0; f < NW; f++) dst[f] = W.ras + c0 + (int64_t)f * K * S;
+  remap_walk_pi<METHOD, NW, PD, false, false>(P, V, c0, m, g0, K, zx, pi, K, lcm, po, K, dst);
+#pragma unroll
+  for (int f = 0; f < NW; f++) {
+    const int g = g0 + f;
+    double* fp = (g == 0) ? V.temp + c0 + lev4(V, 1, m)
+               : (g == 1) ? V.saln + c0 + lev4(V, 1, m)
+                          : V.tracer + c0 + levt(V, 1, m, g - 1);
+    stage_column(fp, 0, dst[f], S, K, S);
+  }
+}
+
+template <int METHOD, int R, int PD, class PO>
+HYB_HDI void ra_rest(const KParams& P, const View& V, const Scratch& W, int64_t c0, int m,
+                     int g0, int r, int K, double zx, const PiRa& pi, uint64_t lcm, PO& po,
+                     bool staged) {
+  if constexpr (R >= 1) {
+    if (r == R)
+      ra_walk<METHOD, R, PD>(P, V, W, c0, m, g0, K, zx, pi, lcm, po, staged);
+    else
+      ra_rest<METHOD, R - 1, PD>(P, V, W, c0, m, g0, r, K, zx, pi, lcm, po, staged);
+  }
+}
+
+// ---------------------------------------------------------------------
+// The Robert-Asselin branch of one p-column on the streaming design
+// (hybgencj, hybgen.F90:1909-2274), after hybgenaj has left the level-n
+// interfaces in p.  One pass over the column forms, without writing any
+// field:
+//   * dp0ij and dp0cum from the level-n p (1931-1960) and the fixed layers
+//     from topiso (1962-1970);
+//   * the old level-m interfaces, running sums of dp(m), into the
+//     workspace;
+//   * the filtered thicknesses dp(m)+0.5*ra2fac*(dpo(n)+dp(n)-2*dpo(m))
+//     (2024-2032) and their running sums, the new interfaces, into the
+//     lane's LDS column (levels 2..K; 1 and K+1 in registers);
+//   * the PCM mask (2034-2046);
+//   * the in-place test of the walks (regrid_stream's) and the range test
+//     of the fast quotients.
+// A column that fails the range test returns false untouched, for
+// column_ra.  Otherwise T, S (with th3d) and the tracers of level m are
+// remapped by walk_core, with the old thicknesses dp(m) as stored (PiRa), no
+// massless fill, and the new grid's K layers (KO = K: its bottom need not
+// be the old one, and the column range covers every old level): in place,
+// or, where an output would overwrite a level the walk has not read (the
+// filtered bottom above the old one with massless layers below, for one),
+// into the workspace W.ras and then over the fields.  Then dp(m) is written
+// and p takes the level-m interfaces (2032).
+// ---------------------------------------------------------------------
+template <int METHOD, int NF, int PD>
+HYB_HDI bool ra_stream(const KParams& P, const View& V, const Scratch& W, int i, int j, int m,
+                       int n, double* row, int ld) {
+  const int K = P.kk;
+  const int64_t c0 = ij_of(V, i, j);
+  const int64_t S = V.NIJ;
+  double* dpm = V.dp + c0 + lev4(V, 1, m);
+  const double* dpn = V.dp + c0 + lev4(V, 1, n);
+  const double* dpon = V.dpo + c0 + lev4(V, 1, n);
+  const double* dpom = V.dpo + c0 + lev4(V, 1, m);
+  const double* th3dm = V.th3d + c0 + lev4(V, 1, m);
+  const double* theta = V.theta + c0;
+  double* pg = V.p + c0;
+  double* pres = W.pres + c0;
+  const double dpthin = kDpthin;
+  // 1931-1960
+  const double qdep = dmax(0.0, dmin(1.0, (V.depths[c0] - P.dsns) / (P.dpns - P.dsns)));
+  const double topiso = V.topiso[c0];
+  double cum = 0.0 + (qdep * P.dp0k[0] + (1.0 - qdep) * P.ds0k[0]);  // dp0cum(2)
+  bool fixed = true;  // k <= fixlay
+  uint64_t lcm = 0;
+  double pr = 0.0, ps = 0.0;  // pres(k), prsf(k)
+  double q0 = 0.0, q1 = 0.0;  // pres(k-2), pres(k-1)
+  double zlast = 0.0;         // the deepest old interface above the bottom
+  bool tame = true, have = false;
+  double cand = 0.0;
+  pres[0] = 0.0;
+  auto level = [&](int k, double dmk, double dold, double dmid, double dnew, double pnk,
+                   double thk, double tak) {
+    if (k >= 2) {
+      const double d = P.dp0k[k - 1];
+      double q;
+      if (d <= P.dp00i) {
+        q = d;
+      } else {
+        q = dmax(P.dp00i, d * d / dmax(d, pnk - cum));
+      }
+      const double dp0 = dmin(q, qdep * d + (1.0 - qdep) * P.ds0k[k - 1]);
+      fixed = fixed && !(cum >= topiso);  // 1962-1970: the loop ends at dp0cum(k) >= topiso
+      cum = cum + dp0;
+    }
+    const double prn = pr + dmk;
+    const double qr = 0.5 * P.ra2fac * (dold + dnew - 2.0 * dmid);
+    const double dk = dmid + qr;
+    const double psn = ps + dk;
+    if (P.isopcm) {
+      const bool v = !fixed && (dmk <= dpthin || dabs(thk - tak) < P.hybiso);
+      if (v) lcm |= 1ull << (k - 1);
+    }
+    tame &= (dmk >= 0.0);
+    tame &= tame_p(prn);
+    tame &= tame_p(psn);
+    // output k ends in old layer <= k-3 only if pres(k-2) >= prsf(k+1)
+    if (k >= 4 && q0 >= psn && (!have || psn < cand)) {
+      cand = psn;
+      have = true;
+    }
+    pres[(int64_t)k * S] = prn;
+    if (prn > pr) zlast = pr;
+    if (k <= K - 1) row[(int64_t)(k - 1) * ld] = psn;  // prsf(k+1) for k+1 in 2..K
+    q0 = q1;
+    q1 = pr;
+    pr = prn;
+    ps = psn;
+  };
+  {
+    // batches of 8 levels, the next batch loaded before the current one is
+    // used (indices clamped into the column; values beyond it unused);
+    // p(k) of level n is read here, before any of p is rewritten
+    constexpr int B = 8;
+    double a0[B], a1[B], a2[B], a3[B], a4[B], a5[B], a6[B];
+    auto load = [&](int k0, double* x0, double* x1, double* x2, double* x3, double* x4,
+                    double* x5, double* x6) {
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        const int k = k0 + q <= K ? k0 + q : K;
+        x0[q] = dpm[(int64_t)(k - 1) * S];
+        x1[q] = dpon[(int64_t)(k - 1) * S];
+        x2[q] = dpom[(int64_t)(k - 1) * S];
+        x3[q] = dpn[(int64_t)(k - 1) * S];
+        x4[q] = pg[(int64_t)(k - 1) * S];
+        x5[q] = th3dm[(int64_t)(k - 1) * S];
+        x6[q] = theta[(int64_t)(k - 1) * S];
+      }
+    };
+    load(1, a0, a1, a2, a3, a4, a5, a6);
+#pragma unroll 1
+    for (int k0 = 1; k0 <= K; k0 += B) {
+      double b0[B], b1[B], b2[B], b3[B], b4[B], b5[B], b6[B];
+      load(k0 + B <= K ? k0 + B : K, b0, b1, b2, b3, b4, b5, b6);
+#pragma unroll
+      for (int q = 0; q < B; q++)
+        if (k0 + q <= K) level(k0 + q, a0[q], a1[q], a2[q], a3[q], a4[q], a5[q], a6[q]);
+#pragma unroll
+      for (int q = 0; q < B; q++) {
+        a0[q] = b0[q];
+        a1[q] = b1[q];
+        a2[q] = b2[q];
+        a3[q] = b3[q];
+        a4[q] = b4[q];
+        a5[q] = b5[q];
+        a6[q] = b6[q];
+      }
+    }
+  }
+  const double zx = pr;  // pres(kk+1)
+  // An output ending above zlast may overwrite a level the walk reads
+  // later.  One ending below it ends in the deepest layer, which reaches the
+  // bottom: the walk has folded every unread level into the column range at
+  // the top of that layer (walk_core), every later output ends there too,
+  // and nothing more is read.  (With the interfaces in order, tame.)
+  const bool staged = have && cand <= zlast;
+  if (!tame || (staged && !W.ras)) return false;
+  StagedInterfaces po{row, ld, K + 1, 0.0, ps};
+  PiRa pi{pres, dpm, S};
+  const int nfields = 2 + P.ntracr;
+  int g0 = 0;
+  for (; g0 + NF <= nfields; g0 += NF)
+    ra_walk<METHOD, NF, PD>(P, V, W, c0, m, g0, K, zx, pi, lcm, po, staged);
+  ra_rest<METHOD, NF - 1, PD>(P, V, W, c0, m, g0, nfields - g0, K, zx, pi, lcm, po, staged);
+  // 2024-2032: dp(m) and the level-m interfaces
+  double sum = 0.0;
+#pragma unroll 4
+  for (int k = 1; k <= K; k++) {
+    const double qr = 0.5 * P.ra2fac * (dpon[(int64_t)(k - 1) * S] + dpn[(int64_t)(k - 1) * S] -
+                                        2.0 * dpom[(int64_t)(k - 1) * S]);
+    const double dk = dpom[(int64_t)(k - 1) * S] + qr;
+    dpm[(int64_t)(k - 1) * S] = dk;
+    sum = sum + dk;
+    pg[(int64_t)k * S] = sum;
+  }
+  return true;
+}
+
 
 // ---------------------------------------------------------------------
 // Mixed-precision remap (HYB_PREC_MIXED; no reference counterpart): the
@@ -1822,7 +2076,9 @@ HYB_HDI bool uv_stream(const KParams& P, const View& V, int i, int j, int nl, bo
       have = true;
     }
     const double q3 = q2 + dk;
-    tame = tame & (dk >= 0.0) & tame_p(q3) & tame_p(pr);
+    tame &= (dk >= 0.0);
+    tame &= tame_p(q3);
+    tame &= tame_p(pr);
     q0 = q1;
     q1 = q2;
     q2 = q3;

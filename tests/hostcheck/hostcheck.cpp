@@ -84,13 +84,15 @@ static int64_t g_unsafe_cols = 0;
 static int g_uv_recompute = 0;  // 1: the read-ahead regrid, remap and u/v variants
 
 static int64_t g_uv_fallback = 0;
+static int64_t g_ra_fallback = 0;
 
 static void run_stream(const KParams& P, const View& V, int m, int n, int raflag) {
   const int ii = P.ii, jj = P.jj, kk = P.kk;
   std::vector<double> pres((size_t)V.NIJ * (kk + 1));
   std::vector<int32_t> kpv((size_t)V.NIJ);
   std::vector<uint64_t> lcmv((size_t)V.NIJ);
-  Scratch W{pres.data(), kpv.data(), lcmv.data()};
+  std::vector<double> rasv(raflag ? (size_t)V.NIJ * kk * 3 : 0);
+  Scratch W{pres.data(), kpv.data(), lcmv.data(), nullptr, raflag ? rasv.data() : nullptr};
   const int itr[3] = {P.itr801, P.itr802, P.itr803};
   double* src[3] = {V.dp, V.temp, V.saln};
   for (int d = 0; d < 3; d++)
@@ -162,10 +164,26 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
   for (int pass = 0; pass <= (raflag ? 1 : 0); pass++) {
     int nl = pass == 0 ? n : m;
     if (pass == 1) {
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int j = 1; j <= jj; j++)
-        for (int i = 1; i <= ii; i++)
-          if (V.ip[ij_of(V, i, j)]) column_ra<64>(P, V, i, j, m, n, row_lcm(P));
+      int64_t ra_fb = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : ra_fb)
+      for (int j = 1; j <= jj; j++) {
+        double row[kKMax + 2];
+        for (int i = 1; i <= ii; i++) {
+          if (!V.ip[ij_of(V, i, j)]) continue;
+          bool ok;
+          switch (P.hybmap) {
+            case 0: ok = ra_stream<0, 3, 2>(P, V, W, i, j, m, n, row, 1); break;
+            case 1: ok = ra_stream<1, 3, 2>(P, V, W, i, j, m, n, row, 1); break;
+            case 2: ok = ra_stream<2, 3, 2>(P, V, W, i, j, m, n, row, 1); break;
+            default: ok = ra_stream<3, 3, 2>(P, V, W, i, j, m, n, row, 1); break;
+          }
+          if (!ok) {
+            ra_fb++;
+            column_ra<64>(P, V, i, j, m, n, row_lcm(P));
+          }
+        }
+      }
+      g_ra_fallback = ra_fb;
     }
     for (int k = 2; k <= kk + 1; k++)
       for (int i = 1; i <= ii; i++) p_halo_jrow(P, V, i, k);
@@ -229,6 +247,8 @@ extern "C" int hostcheck_hybgen_stream_rc(const hyb_params* P, const hyb_fields*
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }
 
 extern "C" int64_t hostcheck_stream_uv_fallback_points(void) { return g_uv_fallback; }
+
+extern "C" int64_t hostcheck_stream_ra_fallback_columns(void) { return g_ra_fallback; }
 
 // ---- one column through the streaming walks (column-level checks) ----
 template <bool MIXED, int M>
