@@ -776,10 +776,10 @@ extern "C" {
 
 size_t hycom_hybgen_noalloc_capacity(void) { return kNoallocBytes; }
 
-size_t hycom_hybgen_noalloc_need(const hyb_params* P) {
+size_t hycom_hybgen_noalloc_need(const hyb_params* P, int raflag) {
   if (!P) return 0;
   const Shape s = shape_of(P);
-  return noalloc_need(s.NIJ, s.K);
+  return noalloc_need(s.NIJ, s.K, raflag != 0);
 }
 
 int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m, int n,

@@ -131,7 +131,7 @@ SIGNATURES = {
     "hycom_hybgen_device_noalloc": (ctypes.c_int, [_P(HybParams), _P(HybFields), ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "hycom_hybgen_noalloc_capacity": (ctypes.c_size_t, []),
-    "hycom_hybgen_noalloc_need": (ctypes.c_size_t, [_P(HybParams)]),
+    "hycom_hybgen_noalloc_need": (ctypes.c_size_t, [_P(HybParams), ctypes.c_int]),
     "hycom_hybgen_record_decisions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "hycom_hybgen_selftest_quotients": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 4
                                         + [ctypes.c_int64, ctypes.c_void_p]),

@@ -188,8 +188,9 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
  * the reference: it is for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  The streaming kernels' workspace (old interfaces,
- * kp, PCM mask and u/v fallback flags: hycom_hybgen_noalloc_need() bytes,
- * NIJ*(8*(kdm+1) + 20) plus alignment) comes from one fixed device array
+ * kp, PCM mask and u/v fallback flags, plus the Robert-Asselin outputs
+ * with hybgen_raflag: hycom_hybgen_noalloc_need() bytes, NIJ*(8*(kdm+1) +
+ * 20), and 24*NIJ*kdm more with raflag, plus alignment) comes from one fixed device array
  * of hycom_hybgen_noalloc_capacity() bytes (256 MiB) per GPU, held from the
  * library's load until the process ends; larger domains fail with
  * HYB_EINVAL.  Nothing of it needs to start at zero: every word a kernel
@@ -202,7 +203,7 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F,
 int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F,
                                 int m, int n, int raflag, void* stream);
 size_t hycom_hybgen_noalloc_capacity(void);
-size_t hycom_hybgen_noalloc_need(const hyb_params* P);
+size_t hycom_hybgen_noalloc_need(const hyb_params* P, int raflag);
 
 /* Per-column decisions of the last hycom_hybgen_device() call on this
  * thread's library context, for decision-parity checks: fixlay and kp of

@@ -120,14 +120,15 @@ def test_noalloc_entry_sizes_and_no_gpu():
     NIJ = (16 + 12) * (12 + 12)
     r = lambda b: (b + 255) // 256 * 256  # noqa: E731
     want = r(8 * NIJ * (P.kdm + 1)) + r(4 * NIJ) + r(8 * NIJ) + r(8 * NIJ)
-    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(P)) == want
+    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(P), 0) == want
+    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(P), 1) == want + r(24 * NIJ * P.kdm)
     d = abi.HostDomain(P)
     F = d.fields()
     assert lib.hycom_hybgen_device_noalloc(ctypes.byref(P), ctypes.byref(F), 1, 2, 0,
                                            None) == abi.HYB_ENODEVICE
     # a domain whose workspace exceeds the array: refused by its shape alone
     big = _params(idm=1200, jdm=900, kdm=41)
-    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(big)) > cap
+    assert lib.hycom_hybgen_noalloc_need(ctypes.byref(big), 0) > cap
     assert lib.hycom_hybgen_device_noalloc(ctypes.byref(big), ctypes.byref(F), 1, 2, 0,
                                            None) == -1  # HYB_EINVAL
     assert b"fixed array" in lib.hycom_hybgen_last_error()

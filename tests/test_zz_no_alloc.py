@@ -42,7 +42,7 @@ def test_noalloc_matches_device_call_twice(gpu):
     assert cap == 256 << 20
     for name, wl, over, seed in CASES:
         P, w = abi.workload_params(wl, **over)
-        need = lib.hycom_hybgen_noalloc_need(ctypes.byref(P))
+        need = lib.hycom_hybgen_noalloc_need(ctypes.byref(P), 0)
         print(f"\n[case] noalloc {name} {over} seed={seed} workspace={need} B", flush=True)
         assert 0 < need <= cap
         want, got = _pair(P, w, seed)
