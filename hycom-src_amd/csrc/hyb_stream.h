@@ -1256,6 +1256,9 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   double s0[NF], s1[NF], s2[NF], s3[NF];  // s(l-1), s(l), s(l+1), s(l+2)
   double smin[NF], smax[NF], skp[NF], sfirst[NF];
   double acc[NF], ov[NF], prev[NF];
+  // FAST (the mixed form): each output's sum of piece integrals is
+  // compensated (TwoSum: acc + accc), as the sums that set conservation
+  double accc[NF];
   CoefState<CR, NF> cst;                       // reconstruction state
   RawT sq[NF][PQ];                             // s(l+3..l+2+PD), loaded ahead
   double prq[PQ];                              // raw(l+3..l+2+PD) of pi
@@ -1294,7 +1297,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
   for (int f = 0; f < NF; f++) {
     s0[f] = s1[f] = s2[f] = s3[f] = 0.0;
     smin[f] = smax[f] = skp[f] = sfirst[f] = 0.0;
-    acc[f] = ov[f] = prev[f] = 0.0;
+    acc[f] = ov[f] = prev[f] = accc[f] = 0.0;
     cst.asc[f] = cst.alc[f] = 0.0;
     cst.wc1[f] = cst.wc2[f] = cst.wz1[f] = cst.wz2[f] = cst.zw3l[f] = 0.0;
     if (f < nf) {
@@ -1467,6 +1470,13 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
       if constexpr (FD64) return div3(x);
       return FAST ? x * (1.0 / 3.0) : x / 3.0;
     };
+    // hi + lo += x exactly (Knuth's TwoSum)
+    auto tsum = [](double& hi, double& lo, double x) {
+      const double t = hi + x;
+      const double bp = t - hi;
+      lo = lo + ((hi - (t - bp)) + (x - bp));
+      hi = t;
+    };
     // Every output still to come ends in this layer when it reaches the
     // bottom, and the walk stops here: fold the levels not read yet into
     // the column range now, before any of them is overwritten.
@@ -1501,20 +1511,26 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
           for (int f = 0; f < NF; f++) {
             v[f] = 0.0;
             if (f < nf) {
-              double a = acc[f];
               const double o = ov[f];
+              double t;  // the piece of layer l
               if (METHOD == 0) {
-                a = a + dpb * (s1[f] - o);
+                t = dpb * (s1[f] - o);
               } else if (METHOD == 1) {
                 const double cc0 = s1[f] - o - 0.5 * c1l[f];
-                a = a + dprsl * (cc0 * qb0 + c1l[f] * qb1);
+                t = dprsl * (cc0 * qb0 + c1l[f] * qb1);
               } else if (METHOD == 2) {
-                a = a + dprsl * ((c1l[f] - o) * qb0 + (c2l[f] + c3l[f]) * qb1 - c3l[f] * qb2);
+                t = dprsl * ((c1l[f] - o) * qb0 + (c2l[f] + c3l[f]) * qb1 - c3l[f] * qb2);
               } else {
-                a = a + dpb * (qb0 * (s1[f] - o) + qb1 * (c1l[f] - o) + qb2 * (c2l[f] - o));
+                t = dpb * (qb0 * (s1[f] - o) + qb1 * (c1l[f] - o) + qb2 * (c2l[f] - o));
               }
-              if constexpr (FD64) v[f] = o + qdiv_any(a, rz);
-              else v[f] = FAST ? o + a * rdz : o + a / dz;
+              if constexpr (FAST) {
+                tsum(acc[f], accc[f], t);
+                v[f] = o + (acc[f] + accc[f]) * rdz;
+              } else {
+                const double a = acc[f] + t;
+                if constexpr (FD64) v[f] = o + qdiv_any(a, rz);
+                else v[f] = o + a / dz;
+              }
             }
           }
         }
@@ -1524,8 +1540,10 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
         accon = false;
       } else if (!acc_thin) {
 #pragma unroll
-        for (int f = 0; f < NF; f++)
-          if (f < nf) acc[f] = acc[f] + dprsl * (s1[f] - ov[f]);
+        for (int f = 0; f < NF; f++) {
+          if (FAST && f < nf) tsum(acc[f], accc[f], dprsl * (s1[f] - ov[f]));
+          else if (f < nf) acc[f] = acc[f] + dprsl * (s1[f] - ov[f]);
+        }
       }
     }
     // ------------------------------------------ outputs starting in layer l
@@ -1635,6 +1653,7 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
                 else if (kp <= l + 2) o = skp[f];
                 else o = static_cast<double>(fp[f][(int64_t)((mid < kp ? mid : kp) - 1) * S]);
                 ov[f] = o;
+                accc[f] = 0.0;
                 if (METHOD == 0) {
                   acc[f] = dpt * (s1[f] - o);
                 } else if (METHOD == 1) {

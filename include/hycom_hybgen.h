@@ -55,8 +55,11 @@ extern "C" {
 
 /* arithmetic modes */
 #define HYB_PREC_FP64   0  /* FP64 throughout, bit-faithful to the reference   */
-#define HYB_PREC_MIXED  1  /* FP32 reconstruction/remap, FP64 interfaces and
-                              compensated overlap sums                          */
+#define HYB_PREC_MIXED  1  /* FP32 reconstruction (edge values, slopes,
+                              limiters, grid metrics, hardware reciprocals);
+                              FP64 interfaces, FP64 piece overlap integrals
+                              and TwoSum-compensated per-output sums; the
+                              regrid stays bitwise FP64                         */
 
 /*
  * Run-time parameters of one hybgen call.  Every member is the value of the
