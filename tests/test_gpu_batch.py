@@ -15,7 +15,7 @@ import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
 
-# (idm, jdm, kdm, hybmap, precision, nreg, ntracr, extreme, seed)
+# (idm, jdm, kdm, hybmap, precision, nreg, ntracr, extreme, seed[, ldenfix])
 RAGGED = [
     (100, 100, 41, 2, 0, 1, 4, 0, 1),
     (37, 211, 22, 0, 0, 0, 1, 0, 2),
@@ -26,14 +26,17 @@ RAGGED = [
     (70, 120, 41, 3, 0, 0, 4, 1, 7),
     (45, 33, 32, 1, 1, 0, 1, 0, 8),
     (210, 52, 22, 3, 0, 1, 2, 0, 9),
+    (90, 70, 41, 2, 0, 0, 2, 0, 10, 1),  # HYBGEN_DENFIX
+    (80, 90, 32, 3, 1, 1, 1, 0, 11, 1),  # HYBGEN_DENFIX, mixed
 ]
 
 
 def _domain(spec, n=2):
-    idm, jdm, kdm, hybmap, prec, nreg, ntr, extreme, seed = spec
+    idm, jdm, kdm, hybmap, prec, nreg, ntr, extreme, seed = spec[:9]
+    denfix = spec[9] if len(spec) > 9 else 0
     dx = dict(dx00=30.0, dx00f=1.1, dx00x=300.0) if extreme else {}
     P, w = abi.workload_params("cfg1", idm=idm, jdm=jdm, kdm=kdm, hybmap=hybmap,
-                               nsigma=8, nreg=nreg, ntracr=ntr,
+                               nsigma=8, nreg=nreg, ntracr=ntr, ldenfix=denfix,
                                trcflg=[2, 0, 0, 0][:ntr], **dx)
     d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=seed, extreme=extreme),
                                  m=3 - n, n=n)
