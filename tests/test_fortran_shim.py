@@ -87,3 +87,68 @@ def test_model_call_through_shim_on_gpu(gpu, raflag):
     assert rc == 0
     df = ol.diff_domains(want, got)
     assert not df, f"model call through the shim differs from the oracle: {df}"
+
+
+def _dynamics(d, m, n, step):
+    """A deterministic stand-in for one model step between hybgen calls,
+    the same on both sides of the A/B: level n takes level m's state with
+    the interfaces displaced (ordered, the bottom kept), T, S, tracers,
+    u and v perturbed smoothly, th3d recomputed from T and S, and dpo set
+    as the Robert-Asselin filter expects (dpo(n) = the new thickness,
+    dpo(m) = the current one)."""
+    P = d.P
+    nb, K = P.nbdy, P.kdm
+    rng = np.random.default_rng(1000 + step)
+    a = d.a
+    ip = a["ip"] != 0
+    lm, ln = m - 1, n - 1
+    dp = a["dp"][..., lm].copy()
+    p = np.concatenate([np.zeros(dp.shape[:2] + (1,)), np.cumsum(dp, axis=2)], axis=2)
+    # interior interfaces move by up to 30% of the thinner neighbouring layer
+    lim = 0.3 * np.minimum(dp[..., :-1], dp[..., 1:])
+    p[..., 1:-1] += lim * rng.uniform(-1.0, 1.0, lim.shape)
+    p[..., 1:-1] = np.minimum(np.maximum.accumulate(p[..., 1:-1], axis=2), p[..., -1:])
+    newdp = np.where(ip[..., None], np.diff(p, axis=2), dp)
+    for name in ("temp", "saln", "u", "v"):
+        x = a[name][..., lm]
+        a[name][..., ln] = x + 0.01 * np.sin(0.3 * np.arange(x.shape[0])[:, None, None] + step)
+    if P.ntracr:
+        t = a["tracer"][..., lm, :]
+        a["tracer"][..., ln, :] = np.maximum(t * (1.0 + 0.001 * step), 0.0)
+    a["dp"][..., ln] = newdp
+    a["dpu"][..., ln] = a["dpu"][..., lm]
+    a["dpv"][..., ln] = a["dpv"][..., lm]
+    sig = np.vectorize(lambda t, s: ol.ref(False).href_sig(t, s))
+    th = sig(a["temp"][..., ln], a["saln"][..., ln]) - P.thbase
+    a["th3d"][..., ln] = np.where(ip[..., None], th, a["th3d"][..., ln])
+    a["dpo"][..., ln] = newdp
+    a["dpo"][..., lm] = dp
+
+
+@pytest.mark.gpu
+def test_model_steps_through_shim_on_gpu(gpu):
+    """An A/B over 8 model steps at the model's call site: one state steps
+    with the reference hybgen (HYCOM-src's hybgen.F90, flang), the other
+    with hybgen resolved to the shim (the HIP library in copy mode), each
+    evolving on its own through the same stand-in dynamics, leapfrog time
+    levels and the Robert-Asselin branch on, as mod_hycom.F90:2650 calls
+    it; the two states are compared bit for bit after every call."""
+    print("\n[case] 8-step A/B, reference hybgen vs shim", flush=True)
+    d = _domain(hybmap=2, ntracr=3, trcflg=[2, 0, 0])
+    ref_lib = ol.ref(False)
+    ref_lib.href_sig.restype = ctypes.c_double
+    ref_lib.href_sig.argtypes = [ctypes.c_double, ctypes.c_double]
+    a, b = d.copy(), d.copy()
+    secs = ctypes.c_double()
+    for step in range(8):
+        m, n = (1, 2) if step % 2 == 0 else (2, 1)
+        _dynamics(a, m, n, step)
+        _dynamics(b, m, n, step)
+        assert not ol.diff_domains(a, b, list(abi.FIELD_NAMES))
+        print(f"[step] {step} m={m} n={n}", flush=True)
+        ol.run_ref(a, m=m, n=n, raflag=1)
+        rc = shim().href_hybgen(ctypes.byref(b.P), ctypes.byref(b.fields()), m, n, 1,
+                                ctypes.byref(secs))
+        assert rc == 0
+        df = ol.diff_domains(a, b)
+        assert not df, f"step {step}: reference and shim differ: {df}"

@@ -116,3 +116,21 @@ def test_oracle_fixed_layers_agree_with_reference_output(seed, hybmap):
             assert p[i, j, k] == min(dp0cum[k], p[i, j, K]), (i, j, k, int(fx[i, j]))
             checked += 1
     assert checked > 10 * sea.sum()
+
+
+def test_multi_step_oracle_matches_reference():
+    """The oracle against the reference hybgen over 8 model steps, each
+    state evolving on its own through the stand-in dynamics of the shim
+    A/B (leapfrog time levels, Robert-Asselin branch on), bit for bit after
+    every call."""
+    import test_fortran_shim as T
+    d = T._domain(hybmap=3, ntracr=3, trcflg=[2, 0, 0])
+    a, b = d.copy(), d.copy()
+    for step in range(8):
+        m, n = (1, 2) if step % 2 == 0 else (2, 1)
+        T._dynamics(a, m, n, step)
+        T._dynamics(b, m, n, step)
+        ol.run_ref(a, m=m, n=n, raflag=1)
+        ol.run_oracle(b, m=m, n=n, raflag=1)
+        df = ol.diff_domains(a, b)
+        assert not df, f"step {step}: {df}"
