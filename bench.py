@@ -539,6 +539,8 @@ def main():
     ap.add_argument("--batch-baseline", action="store_true",
                     help="also time the batch's domains one hycom_hybgen_device call each")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the HBM-counter calibration kernels (1 GiB fill, sum, clone)")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-mode leg")
     ap.add_argument("--no-denfix", action="store_true", help="skip the HYBGEN_DENFIX leg")
     ap.add_argument("--no-raflag", action="store_true", help="skip the Robert-Asselin leg")
@@ -571,6 +573,23 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate:
+        # known traffic for the FETCH_SIZE / WRITE_SIZE factors: a write-only
+        # fill, a read-only sum and a copy, 1 GiB each, 3 times
+        n = 1 << 27
+        x = torch.empty(n, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            x.fill_(1.0)
+            torch.cuda.synchronize()
+            s_ = x.sum()
+            torch.cuda.synchronize()
+            y = x.clone()
+            torch.cuda.synchronize()
+            del y
+        print(json.dumps({"calibration": "1 GiB float64: fill (writes 1 GiB), sum (reads 1 GiB), "
+                                         "clone (reads and writes 1 GiB), 3 times",
+                          "sum": float(s_.item())}), flush=True)
+        return
     lib = abi.load()
     abi.check(lib.hycom_hybgen_set_kernel_path(args.kernel_path))
     abi.check(lib.hycom_hybgen_tune(0, args.remap_variant))
