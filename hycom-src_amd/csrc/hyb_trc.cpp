@@ -38,31 +38,13 @@ int64_t nij_of(const hyb_trc_params* P) {
   return ((int64_t)P->idm + 2 * P->nbdy) * ((int64_t)P->jdm + 2 * P->nbdy);
 }
 
-// slopes workspace (NIJ*ki*kTrcChunk doubles) and the first-bad-column word, kept
+// the first-bad-column word, kept
 std::mutex g_ws_mu;
-double* g_slope = nullptr;
-size_t g_slope_n = 0;
 int32_t* g_bad = nullptr;
 
-int workspace(size_t n, double** slope, int32_t** bad) {
-  if (!g_bad) HIPCHK(hipMalloc(&g_bad, sizeof(int32_t)));
-  if (n > g_slope_n) {
-    if (g_slope) HIPCHK(hipFree(g_slope));
-    g_slope = nullptr;
-    g_slope_n = 0;
-    HIPCHK(hipMalloc(&g_slope, n * sizeof(double)));
-    g_slope_n = n;
-  }
-  *slope = g_slope;
-  *bad = g_bad;
-  return 0;
-}
-
 int enqueue(const hyb_trc_params* P, const hyb_trc_fields* F, hipStream_t s, int32_t** bad_out) {
-  double* slope = nullptr;
-  int32_t* bad = nullptr;
-  int rc = workspace((size_t)nij_of(P) * P->ki * kTrcChunk, &slope, &bad);
-  if (rc) return rc;
+  if (!g_bad) HIPCHK(hipMalloc(&g_bad, sizeof(int32_t)));
+  int32_t* bad = g_bad;
   KTrc Q;
   memset(&Q, 0, sizeof Q);
   Q.ii = P->ii;
@@ -79,7 +61,6 @@ int enqueue(const hyb_trc_params* P, const hyb_trc_fields* F, hipStream_t s, int
   Q.po = F->po;
   Q.mask = F->mask;
   Q.so = F->so;
-  Q.slope = slope;
   Q.bad = bad;
   HIPCHK(launch_trc(Q, s));
   if (bad_out) *bad_out = bad;

@@ -341,8 +341,6 @@ extern "C" int hostcheck_trc(const hyb_trc_params* P, const hyb_trc_fields* F) {
   Q.po = F->po;
   Q.mask = F->mask;
   Q.so = F->so;
-  std::vector<double> slope((size_t)Q.NIJ * Q.ki * kTrcChunk, 0.0);
-  Q.slope = slope.data();
   int bad = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
   for (int j = 1; j <= Q.jj; j++)
