@@ -224,27 +224,22 @@ int hycom_hybgen_selftest_quotients(int which, const double* a, const double* b,
                                     double* fast, double* ieee, int64_t n, void* stream);
 
 /* Kernel selection (testing and measurement; no reference counterpart).
- * 0: automatic (streaming kernels when nhybrd==kdm and no HYBGEN_DENFIX,
- * per-lane-array kernels otherwise), 1: per-lane-array kernels only,
- * 2: register-resident kernels (kdm in {22,32,41} only), 3: streaming
- * kernels only (HYB_EUNSUPPORTED where a family does not apply).
+ * 0: automatic (streaming kernels when nhybrd==kdm, per-lane-array kernels
+ * otherwise), 1: per-lane-array kernels only, 2: removed (HYB_EUNSUPPORTED),
+ * 3: streaming kernels only (HYB_EUNSUPPORTED where they do not apply).
  * Process-wide. */
 int hycom_hybgen_set_kernel_path(int path);
 
 /* Variants of the streaming kernels (measurement and tuning; no reference
  * counterpart; results are bitwise the same for every value).  knob 0:
- * p-column remap, value 0..5 = (fields per walk, levels loaded ahead)
- * (6,0) (6,2) (3,0) (3,2) (2,2) (2,4), 6..13 the same walk with the new
- * interfaces read from p instead of LDS: (3,0) (3,0) (2,0) (2,2) (2,0)
- * (3,2) (3,2) (2,2) at 1, 3, 3, 3, 4, 1, 3, 4 waves per SIMD asked of the
- * compiler (6..13), 14 = (3,4) and 15 = (3,2) with non-temporal output
- * stores, both staged in LDS, 16 = the tiled layer-parallel remap (PPM in
- * FP64 with 4 <= kdm <= 63; otherwise 3); knob 1: levels loaded ahead by the
- * u/v walk, 0, 2 or 4 (the default); knob 2: the u/v walk's new thicknesses kept in LDS (0)
- * or formed again from p as the walk needs them (1, no LDS); knob 3: the
- * regrid's window loaded one iteration ahead (1) or not (0); knob 4: the
- * u/v phase on a second stream of the library's, beside the p-column remap
- * (1), or after it on the caller's stream (0).  Process-wide. */
+ * p-column remap variant, only 3 is built (3 fields per walk, 2 levels
+ * loaded ahead, new interfaces staged in LDS); knob 1: levels loaded ahead
+ * by the u/v walk, 0, 2 or 4 (the default); knob 2: the u/v walk's new
+ * thicknesses kept in LDS (0) or formed again from p as the walk needs them
+ * (1, the default, no LDS); knob 3: the regrid's window loaded one
+ * iteration ahead (1, the default) or not (0); knob 4: the u/v phase on a
+ * second stream of the library's, beside the p-column remap (1), or after
+ * it on the caller's stream (0, the default).  Process-wide. */
 int hycom_hybgen_tune(int knob, int value);
 
 /* Measurement hooks (no reference counterpart; the reference's only timer
@@ -439,9 +434,9 @@ int hycom_stokes_vertical_device(const hyb_stokes_params* P, const hyb_stokes_fi
  * tracer relaxation of thermf (thermf.F90:770-806) call them per sea
  * column: si(ki,ks) on the interfaces pi(ki+1) onto so(ko,ks) on po(ko+1).
  * Interfaces are non-decreasing, pi(1) = 0 = po(1).  Where the reference
- * would stop ('bad lf', trcupd.F90:866-869: po out of order) the column's
- * later outputs are NaN and the call returns HYB_EINVAL (host call) after
- * writing the rest.  A thin first output (po(2)-po(1) < 1e-6) takes si(1);
+ * would stop ('bad lf', trcupd.F90:866-869: po out of order) that output
+ * is NaN (thin outputs below it copy it) and the call returns HYB_EINVAL
+ * (host call) after writing the rest.  A thin first output (po(2)-po(1) < 1e-6) takes si(1);
  * the reference reads outside its array there.
  * ------------------------------------------------------------------- */
 typedef struct hyb_trc_params {
@@ -554,8 +549,9 @@ typedef struct hyb_start_fields {
 int hycom_state_start(const hyb_params* P, const hyb_start_fields* F);
 
 /* Same on device pointers, ordered on `stream`; returns after enqueueing
- * (a column that would stop is left NaN, not reported).  Calls on one
- * device share a workspace: do not overlap them on different streams. */
+ * (the output where the reference would stop is left NaN, not reported).
+ * Calls on one device share the first-bad-column word: do not overlap them
+ * on different streams. */
 int hycom_trc_remap_device(const hyb_trc_params* P, const hyb_trc_fields* F, void* stream);
 
 #ifdef __cplusplus

@@ -580,9 +580,10 @@ static void trc_plmtrcs(const double* rl, const double* rr, const double* a, dou
 
 /* pcmtrc (method 0, trcupd.F90:808-909) or plmtrc (method 1, 911-1036)
  * of one column: si[f*ki + l-1], pi[ki+1], po[ko+1], so[f*ko + k-1].
- * Returns -1 where the reference stops ('bad lf'); the outputs from there
- * on are NaN.  A thin first output takes si(1): the reference reads so(0)
- * there, outside the array. */
+ * Returns -1 where the reference stops ('bad lf'); that output is NaN (and
+ * thin outputs below it copy it), later outputs are formed as usual.  A
+ * thin first output takes si(1): the reference reads so(0) there, outside
+ * the array. */
 int oracle_trc_column(int method, int ki, int ks, const double* si, const double* pi,
                       const double* po, int ko, double* so) {
   double p[KD], pit[KD], ql[KD], qr[KD], a[KD], sis[KD], sok[HYB_TRMAX + 5];
