@@ -421,8 +421,10 @@ def batch_leg(args):
         lat.append(time.perf_counter() - ts)
     t_batch = (time.perf_counter() - t0) / args.steps
     rc, gc, tuned = batch.config()
-    print(f"[bench] batch {1e3 * t_batch:.2f} ms/call; widths remap {rc} regrid {gc}",
-          file=sys.stderr, flush=True)
+    ns, kmax = batch.staged_levels()
+    cands = batch.tuning()
+    print(f"[bench] batch {1e3 * t_batch:.2f} ms/call; widths remap {rc} regrid {gc}, "
+          f"{ns} of {kmax - 1} interface levels staged", file=sys.stderr, flush=True)
     lat.sort()
     shapes = [f"{dd.P.idm}x{dd.P.jdm}x{dd.P.kdm}" for dd in doms]
     del batch, doms
@@ -432,8 +434,11 @@ def batch_leg(args):
             "one_call_per_domain": ({"value": layers / t_single, "ms_per_step": 1e3 * t_single,
                                      "speedup_of_batch": t_single / t_batch}
                                     if t_single else None),
-            "autotuned": {"remap_cols": rc, "regrid_cols": gc, "tuned": tuned,
-                          "tuning_calls": calls},
+            "autotuned": {"remap_cols": rc, "regrid_cols": gc, "remap_staged_levels": ns,
+                          "kmax": kmax, "tuned": tuned, "tuning_calls": calls,
+                          "candidates_ms": [{"remap_cols": a, "staged_levels": b,
+                                             "regrid_cols": c, "ms": t}
+                                            for a, b, c, t in cands]},
             "latency_ms": {"p50": 1e3 * lat[len(lat) // 2], "p99": 1e3 * lat[-1]},
             "shapes": shapes}
 

@@ -482,11 +482,17 @@ struct BatchPlan {
   hipStream_t cap = nullptr;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};  // per time level n
-  // online autotuning
-  std::vector<std::pair<int, int>> cands;  // (remap cols, regrid cols)
+  // online autotuning: first the remap workgroups (columns, staged
+  // interface levels) with 256-column regrid workgroups, then the regrid
+  // widths with the best remap candidate
+  struct Cand {
+    int rc, ns, gc;  // remap columns, staged levels, regrid columns
+  };
+  std::vector<Cand> cands;
   std::vector<float> ms;  // best time of each candidate
   int cand = 0;           // candidate measured by the pending events
   int rep = 0;            // measurements of it so far
+  bool second = false;    // the regrid widths are being searched
   bool pending = false, tuned = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t done = nullptr;  // after the last launch of this plan's graphs
@@ -582,21 +588,27 @@ int plan_build(BatchPlan& b, const hyb_domain* D, int ndom, std::vector<unsigned
   b.B = BatchDevice{b.dP, b.dV, b.dW};
   int kmax = 0;
   for (int d = 0; d < ndom; d++) kmax = b.hP[d].kk > kmax ? b.hP[d].kk : kmax;
-  // candidate widths; a remap workgroup holds levels 2..kmax of its columns'
-  // interfaces in LDS, (kmax-1) x cols doubles
+  // remap candidates: a workgroup stages ns interface levels of each of
+  // its columns in LDS (ns x cols doubles, at most 64 KB): all kmax-1, or
+  // 24 or 12 with the deeper levels read from p as the walk reaches them
   b.cands.clear();
   const int lvl = kmax > 1 ? kmax - 1 : 1;
   for (int rc : {64, 128, 256})
-    if (sizeof(double) * lvl * rc <= 65536)
-      for (int gc : {256, 128, 64}) b.cands.push_back({rc, gc});
+    for (int ns : {lvl, 24, 12}) {
+      if (ns != lvl && ns >= lvl) continue;  // a split stages fewer levels than all
+      if (sizeof(double) * ns * rc <= 65536) b.cands.push_back({rc, ns, 256});
+    }
   b.ms.assign(b.cands.size(), -1.0f);
   b.cand = 0;
   b.rep = 0;
+  b.second = false;
   b.pending = false;
   b.tuned = !g_batch_autotune;
-  // first candidate (and the untuned widths): 64-column remaps, 256-column regrid
-  b.L.remap_cols = b.cands[0].first;
-  b.L.regrid_cols = b.cands[0].second;
+  // first candidate (and the untuned widths): 64-column remaps staging every
+  // level, 256-column regrid
+  b.L.remap_cols = b.cands[0].rc;
+  b.L.remap_stage = b.cands[0].ns;
+  b.L.regrid_cols = b.cands[0].gc;
   if (!b.ev0) HIPCHK(hipEventCreate(&b.ev0));
   if (!b.ev1) HIPCHK(hipEventCreate(&b.ev1));
   if (!b.done) HIPCHK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
@@ -652,13 +664,26 @@ int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream
       b.rep = 0;
       int next = ++b.cand;
       if (next >= (int)b.cands.size()) {
-        next = 0;
+        int best = 0;
         for (int q = 1; q < (int)b.cands.size(); q++)
-          if (b.ms[q] < b.ms[next]) next = q;
-        b.tuned = true;
+          if (b.ms[q] < b.ms[best]) best = q;
+        next = best;
+        if (!b.second) {
+          // the regrid widths, with the best remap workgroup
+          b.second = true;
+          const BatchPlan::Cand w = b.cands[best];
+          for (int gc : {128, 64}) {
+            b.cands.push_back({w.rc, w.ns, gc});
+            b.ms.push_back(-1.0f);
+          }
+          next = b.cand;
+        } else {
+          b.tuned = true;
+        }
       }
-      b.L.remap_cols = b.cands[next].first;
-      b.L.regrid_cols = b.cands[next].second;
+      b.L.remap_cols = b.cands[next].rc;
+      b.L.remap_stage = b.cands[next].ns;
+      b.L.regrid_cols = b.cands[next].gc;
       if ((rc = plan_layout(b))) return rc;
     }
   }
@@ -687,6 +712,27 @@ int hycom_hybgen_batch_config(int32_t* remap_cols, int32_t* regrid_cols, int32_t
   if (remap_cols) *remap_cols = g_plan.L.remap_cols;
   if (regrid_cols) *regrid_cols = g_plan.L.regrid_cols;
   if (tuned) *tuned = g_plan.tuned ? 1 : 0;
+  return 0;
+}
+
+int hycom_hybgen_batch_tuning(int32_t* remap_cols, int32_t* staged_levels,
+                              int32_t* regrid_cols, float* ms, int cap) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  if (cap < 0) return fail(HYB_EINVAL, "cap %d", cap);
+  const int nc = (int)g_plan.cands.size();
+  for (int q = 0; q < nc && q < cap; q++) {
+    if (remap_cols) remap_cols[q] = g_plan.cands[q].rc;
+    if (staged_levels) staged_levels[q] = g_plan.cands[q].ns;
+    if (regrid_cols) regrid_cols[q] = g_plan.cands[q].gc;
+    if (ms) ms[q] = g_plan.ms[q];
+  }
+  return nc;
+}
+
+int hycom_hybgen_batch_staged_levels(int32_t* levels, int32_t* kmax) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  if (levels) *levels = g_plan.L.kmax > 0 ? batch_stage_levels(g_plan.L) : 0;
+  if (kmax) *kmax = g_plan.L.kmax;
   return 0;
 }
 

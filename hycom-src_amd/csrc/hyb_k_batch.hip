@@ -99,8 +99,11 @@ kb_regrid(BatchDevice B, BatchGroup G, int n) {
 }
 
 // CW columns per workgroup; each lane's LDS column has stride CW doubles
-template <int METHOD, bool MIXED, int CW>
-__global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int n) {
+// and holds ns levels of the new interfaces: all of them (SPLIT false), or
+// the upper ns with the rest read from p (the split costs the walk a few
+// registers and branches, so the whole column keeps its own instance)
+template <int METHOD, bool MIXED, int CW, bool SPLIT>
+__global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int n, int ns) {
   extern __shared__ double lds[];
   const RowPos r = row_pos(B, G, CW, blockIdx.x);
   const KParams& P = B.P[r.d];
@@ -111,9 +114,15 @@ __global__ void __launch_bounds__(CW) kb_remap(BatchDevice B, BatchGroup G, int 
   const int64_t c0 = ij_of(V, i, r.j);
   const int32_t ipw = V.ip[c0], kpw = W.kp[c0];  // kp of a land column: read, not used
   if ((ipw == 0) | ((kpw & kUnsafe) != 0)) return;
-  StagedInterfaces po = stage_interfaces(lds + threadIdx.x, CW, V.p + c0, V.NIJ, P.kk);
-  if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
-  else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
+  if (SPLIT) {
+    SplitInterfaces po = stage_split(lds + threadIdx.x, CW, V.p + c0, V.NIJ, P.kk, ns);
+    if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
+    else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
+  } else {
+    StagedInterfaces po = stage_interfaces(lds + threadIdx.x, CW, V.p + c0, V.NIJ, P.kk);
+    if (MIXED) remap_mixed<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
+    else remap_stream<METHOD, 3, 2>(P, V, W, i, r.j, n, po);
+  }
 }
 
 __global__ void __launch_bounds__(kWave) kb_remap_fb(BatchDevice B, BatchGroup G, int n) {
@@ -196,28 +205,41 @@ __global__ void __launch_bounds__(kWave) kb_uv_fb(BatchDevice B, BatchGroup G, i
   point_c<64>(P, V, i, j, nl, isv);
 }
 
-template <int CW>
-static void remap_group_cw(const BatchDevice& B, const BatchGroup& G, int n, size_t lds,
+template <int CW, bool SPLIT>
+static void remap_group_cw(const BatchDevice& B, const BatchGroup& G, int n, int ns,
                            hipStream_t s) {
   const dim3 grid(G.nblocks), blk(CW);
+  const size_t lds = sizeof(double) * ns * CW;
   switch (G.key) {  // hybmap + 4 * precision
-    case 0: hipLaunchKernelGGL((kb_remap<0, false, CW>), grid, blk, lds, s, B, G, n); break;
-    case 1: hipLaunchKernelGGL((kb_remap<1, false, CW>), grid, blk, lds, s, B, G, n); break;
-    case 2: hipLaunchKernelGGL((kb_remap<2, false, CW>), grid, blk, lds, s, B, G, n); break;
-    case 3: hipLaunchKernelGGL((kb_remap<3, false, CW>), grid, blk, lds, s, B, G, n); break;
-    case 4: hipLaunchKernelGGL((kb_remap<0, true, CW>), grid, blk, lds, s, B, G, n); break;
-    case 5: hipLaunchKernelGGL((kb_remap<1, true, CW>), grid, blk, lds, s, B, G, n); break;
-    case 6: hipLaunchKernelGGL((kb_remap<2, true, CW>), grid, blk, lds, s, B, G, n); break;
-    default: hipLaunchKernelGGL((kb_remap<3, true, CW>), grid, blk, lds, s, B, G, n); break;
+    case 0: hipLaunchKernelGGL((kb_remap<0, false, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 1: hipLaunchKernelGGL((kb_remap<1, false, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 2: hipLaunchKernelGGL((kb_remap<2, false, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 3: hipLaunchKernelGGL((kb_remap<3, false, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 4: hipLaunchKernelGGL((kb_remap<0, true, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 5: hipLaunchKernelGGL((kb_remap<1, true, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    case 6: hipLaunchKernelGGL((kb_remap<2, true, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
+    default: hipLaunchKernelGGL((kb_remap<3, true, CW, SPLIT>), grid, blk, lds, s, B, G, n, ns); break;
   }
 }
 
-static void remap_group(const BatchDevice& B, const BatchGroup& G, int n, int kmax, int cw,
-                        hipStream_t s) {
-  const size_t lds = sizeof(double) * lds_levels(kmax) * cw;
-  if (cw == 256) remap_group_cw<256>(B, G, n, lds, s);
-  else if (cw == 128) remap_group_cw<128>(B, G, n, lds, s);
-  else remap_group_cw<64>(B, G, n, lds, s);
+// the interface levels a remap workgroup stages per column: remap_stage
+// (0: all, kmax-1), at most kmax-1
+int batch_stage_levels(const BatchLaunch& L) {
+  const int all = lds_levels(L.kmax);
+  return (L.remap_stage > 0 && L.remap_stage < all) ? L.remap_stage : all;
+}
+
+static void remap_group(const BatchDevice& B, const BatchGroup& G, int n, int ns, bool split,
+                        int cw, hipStream_t s) {
+  if (split) {
+    if (cw == 256) remap_group_cw<256, true>(B, G, n, ns, s);
+    else if (cw == 128) remap_group_cw<128, true>(B, G, n, ns, s);
+    else remap_group_cw<64, true>(B, G, n, ns, s);
+  } else {  // every level staged: at most 64 KB with 64 or 128 columns
+    if (cw == 256) remap_group_cw<256, false>(B, G, n, ns, s);
+    else if (cw == 128) remap_group_cw<128, false>(B, G, n, ns, s);
+    else remap_group_cw<64, false>(B, G, n, ns, s);
+  }
 }
 
 static void uv_group(const BatchDevice& B, const BatchGroup& G, int nl, size_t lds,
@@ -243,7 +265,9 @@ hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipSt
     else
       hipLaunchKernelGGL(kb_regrid<64>, grid, dim3(64), 0, s, B, L.regrid, n);
   }
-  for (int q = 0; q < L.nremap; q++) remap_group(B, L.remap[q], n, L.kmax, L.remap_cols, s);
+  const int ns = batch_stage_levels(L);
+  const bool split = ns < lds_levels(L.kmax);
+  for (int q = 0; q < L.nremap; q++) remap_group(B, L.remap[q], n, ns, split, L.remap_cols, s);
   if (L.rows64.nblocks > 0)
     hipLaunchKernelGGL(kb_remap_fb, dim3(L.rows64.nblocks), dim3(kWave), 0, s, B, L.rows64, n);
   if (L.halo_j.nblocks > 0)

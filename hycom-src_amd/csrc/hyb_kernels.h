@@ -63,6 +63,7 @@ struct BatchLaunch {
   int32_t kmax;        // largest kdm of the batch (LDS size)
   int32_t remap_cols;  // columns per remap workgroup: 64, 128 or 256
   int32_t regrid_cols; // columns per regrid workgroup: 64, 128 or 256
+  int32_t remap_stage; // interface levels a remap lane stages in LDS (0: all)
   BatchGroup regrid;                 // row blocks of regrid_cols, all domains
   BatchGroup remap[kMaxGroups];      // row blocks of remap_cols, by hybmap + 4*precision
   int32_t nremap;
@@ -74,6 +75,8 @@ struct BatchLaunch {
 };
 // Enqueues one hybgen(m,n,.false.) step of every domain of the batch.
 hipError_t launch_batch(const BatchDevice& B, const BatchLaunch& L, int n, hipStream_t s);
+// the interface levels per column a remap workgroup of L stages in LDS
+int batch_stage_levels(const BatchLaunch& L);
 
 // NaN/Inf scan of a call's inputs: scan_blocks(P) int32 results in out
 // (hyb_k_misc.hip; the key of a bad column is i*128 + field).

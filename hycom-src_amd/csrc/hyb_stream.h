@@ -816,36 +816,6 @@ struct PoSum {
   }
 };
 
-// New interfaces of a p-column read from p (regrid_stream's output) as the
-// walk asks for them, with the next level always loaded ahead.  A walk's
-// requests are k = 1, 2, 2, 3, ... (non-decreasing by steps of one), so
-// the load of level k+1 has a whole layer iteration to arrive; the next
-// walk over the same column starts again at k = 1.
-struct PoAhead {
-  const double* p;  // level 1 of the column
-  int64_t S;
-  int k1;           // kk+1, the deepest level
-  int k = 1;        // level of cur
-  double cur = 0.0, nxt = 0.0;
-  HYB_HD PoAhead(const double* p_, int64_t S_, int kk) : p(p_), S(S_), k1(kk + 1) {
-    cur = p[0];
-    nxt = p[S];
-  }
-  HYB_HD double operator()(int kq) {
-    if (kq < k) {
-      k = kq;
-      cur = p[(int64_t)(k - 1) * S];
-      nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
-    }
-    while (k < kq) {
-      cur = nxt;
-      k++;
-      nxt = (k < k1) ? p[(int64_t)k * S] : 0.0;
-    }
-    return cur;
-  }
-};
-
 // The same prsf(k) without the LDS column: the new thicknesses are formed
 // again from p at the two neighbouring p-points as the walk asks for them
 // (the operations of the first pass of uv_stream, so the same bits).  The
@@ -1826,6 +1796,56 @@ HYB_HDI StagedInterfaces stage_interfaces(double* row, int ld, const double* pg,
   const double top = pg[0], bot = pg[(int64_t)K * S];
   stage_column(row, ld, pg + S, S, K - 1);
   return StagedInterfaces{row, ld, K + 1, top, bot};
+}
+
+// The new interfaces with only levels 2..ks staged in LDS (ks-1 doubles per
+// lane) and the deeper ones read from p as the walk reaches them, the next
+// one loaded ahead.  A walk's requests are non-decreasing by steps of one
+// and start again at k = 1 for the next walk over the column; while a walk
+// is still in the staged levels, the first deep level and the one after it
+// are already on their way.  With ks = K every level is staged and p is not
+// read again (StagedInterfaces).  The LDS saved lets a workgroup hold more
+// columns, or a CU more waves, for the price of those reads.
+struct SplitInterfaces {
+  const double* base;  // level 2 of the lane's LDS column
+  const double* pg;    // level 1 of the column in p
+  int64_t S;
+  int stride, k1, ks;  // LDS stride; K+1; the deepest staged level
+  double top, bot;     // levels 1 and K+1
+  int k = 0;           // level of cur (0: nothing loaded)
+  double cur = 0.0, nxt = 0.0;
+  HYB_HD double operator()(int kq) {
+    if (kq == 1) return top;
+    if (kq >= k1) return bot;
+    if (kq <= ks) {
+      if (ks + 1 < k1 && k != ks + 1) {  // a new walk: levels ks+1, ks+2 ahead
+        k = ks + 1;
+        cur = pg[(int64_t)ks * S];
+        nxt = pg[(int64_t)(ks + 1) * S];
+      }
+      return base[(int64_t)(kq - 2) * stride];
+    }
+    if (k == 0 || kq < k) {  // (requests that skip the staged levels)
+      k = kq;
+      cur = pg[(int64_t)(kq - 1) * S];
+      nxt = pg[(int64_t)kq * S];
+    }
+    while (k < kq) {  // kq <= K, so level k+1 <= K+1
+      cur = nxt;
+      k++;
+      nxt = pg[(int64_t)k * S];
+    }
+    return cur;
+  }
+};
+// ns levels staged (clamped to 1..K-1)
+HYB_HDI SplitInterfaces stage_split(double* row, int ld, const double* pg, int64_t S, int K,
+                                    int ns) {
+  const int n = ns < K - 1 ? (ns > 1 ? ns : 1) : K - 1;
+  const double top = pg[0], bot = pg[(int64_t)K * S];
+  stage_column(row, ld, pg + S, S, n);
+  SplitInterfaces po{row, pg, S, ld, K + 1, n + 1, top, bot};
+  return po;
 }
 
 // the last RA walk, over r < NF fields (r = 0: none)

@@ -145,6 +145,10 @@ SIGNATURES = {
     "hycom_hybgen_batch_autotune": (ctypes.c_int, [ctypes.c_int]),
     "hycom_hybgen_batch_config": (ctypes.c_int, [_P(ctypes.c_int32), _P(ctypes.c_int32),
                                                  _P(ctypes.c_int32)]),
+    "hycom_hybgen_batch_staged_levels": (ctypes.c_int, [_P(ctypes.c_int32), _P(ctypes.c_int32)]),
+    "hycom_hybgen_batch_tuning": (ctypes.c_int, [_P(ctypes.c_int32), _P(ctypes.c_int32),
+                                                 _P(ctypes.c_int32), _P(ctypes.c_float),
+                                                 ctypes.c_int]),
     "hycom_hybgen_server_create": (ctypes.c_int, [_P(HybServerConfig), _P(ctypes.c_void_p)]),
     "hycom_hybgen_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "hycom_hybgen_server_add_domain": (ctypes.c_int, [ctypes.c_void_p, _P(HybParams),
@@ -428,6 +432,24 @@ class Batch:
         a, b, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         check(load().hycom_hybgen_batch_config(ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         return a.value, b.value, bool(t.value)
+
+    @staticmethod
+    def tuning():
+        """The autotuner's candidates of the last batch: a list of
+        (remap columns, staged levels, regrid columns, ms)."""
+        cap = 64
+        a, b, c = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+        t = (ctypes.c_float * cap)()
+        n = load().hycom_hybgen_batch_tuning(a, b, c, t, cap)
+        check(min(n, 0))
+        return [(a[q], b[q], c[q], round(t[q], 3)) for q in range(min(n, cap))]
+
+    @staticmethod
+    def staged_levels():
+        """(interface levels staged in LDS per remap column, kmax) of the last batch."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        check(load().hycom_hybgen_batch_staged_levels(ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 class Server:

@@ -294,20 +294,37 @@ typedef struct hyb_domain {
  * precision for the remaps), captured once into a HIP graph per batch and
  * replayed; the same domain set on the next call replays the same graph.
  * Each domain's result is bitwise the one of hycom_hybgen_device on that
- * domain alone.  Requires nhybrd == kdm, no HYBGEN_DENFIX, no itracr
- * diagnostics (HYB_EUNSUPPORTED otherwise). */
+ * domain alone.  Requires nhybrd == kdm and no itracr diagnostics
+ * (HYB_EUNSUPPORTED otherwise); HYBGEN_DENFIX, FP64 and mixed domains may
+ * share a batch. */
 int hycom_hybgen_batch(const hyb_domain* D, int ndom, int m, int n, void* stream);
 
-/* Online autotuning of the batch kernels' workgroup widths (columns per
- * workgroup of the regrid and of the remap, which sets the remap's LDS
- * per workgroup): while enabled (the default), the first calls of a new
- * domain set each try one candidate, timed with HIP events, and the
- * fastest is kept for that set.  Results do not depend on the widths. */
+/* Online autotuning of the batch kernels' launch shape: while enabled (the
+ * default), the first calls of a new domain set each try one candidate,
+ * timed with HIP events, and the fastest is kept for that set.  First the
+ * remap workgroups: columns per workgroup (64, 128, 256) times the LDS
+ * split, the interface levels each column stages in LDS (all kmax-1, or
+ * 24 or 12 with the deeper levels read from p as the walk reaches them;
+ * at most 64 KB per workgroup); then the regrid's columns per workgroup
+ * (256, 128, 64) with the best remap shape.  Results do not depend on the
+ * shape. */
 int hycom_hybgen_batch_autotune(int enable);
 
 /* Widths in use for the last batch: columns per workgroup of the remap
  * and of the regrid; *tuned is 1 once the search has finished. */
 int hycom_hybgen_batch_config(int32_t* remap_cols, int32_t* regrid_cols, int32_t* tuned);
+
+/* The LDS split in use for the last batch: interface levels per column
+ * staged in LDS by the remap (*levels) and the batch's largest kdm (*kmax;
+ * kmax-1 levels = all staged). */
+int hycom_hybgen_batch_staged_levels(int32_t* levels, int32_t* kmax);
+
+/* The autotuner's candidates for the last batch, in search order: remap
+ * columns, staged interface levels, regrid columns and the best of the two
+ * timed launches in ms (-1 while not measured), up to cap of them.
+ * Returns the number of candidates (or a negative HYB_E* code). */
+int hycom_hybgen_batch_tuning(int32_t* remap_cols, int32_t* staged_levels,
+                              int32_t* regrid_cols, float* ms, int cap);
 
 /* ---------------------------------------------------------------------- */
 /* Request loop over resident domains (no reference counterpart: serving) */

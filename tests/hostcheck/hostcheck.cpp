@@ -131,8 +131,9 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
         }
         continue;
       }
-      if (g_uv_recompute) {  // the variants that read the new interfaces from p
-        PoAhead pa(V.p + c0, V.NIJ, kk);
+      if (g_uv_recompute) {  // 7 interface levels staged, the rest read from p
+        double srow[kKMax + 2];
+        SplitInterfaces pa = stage_split(srow, 1, V.p + c0, V.NIJ, kk, 7);
         switch (P.hybmap) {
           case 0: remap_stream<0, 2, 0>(P, V, W, i, j, n, pa); break;
           case 1: remap_stream<1, 2, 0>(P, V, W, i, j, n, pa); break;

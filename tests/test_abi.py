@@ -202,3 +202,16 @@ def test_server_config_checks_and_no_gpu():
     assert lib.hycom_hybgen_server_submit(None, 0, 1, 0, 0) == abi.HYB_ENULL
     assert lib.hycom_hybgen_server_poll(None, None, 0, 0, ctypes.byref(n)) == abi.HYB_ENULL
     assert lib.hycom_hybgen_server_destroy(None) == 0
+
+
+def test_batch_tuning_table_without_a_batch():
+    """The autotuner's candidate table is empty before any batch call in
+    this process, and a negative capacity is refused."""
+    lib = abi.load()
+    if lib.hycom_hybgen_batch_tuning(None, None, None, None, 0) > 0:
+        pytest.skip("a batch ran earlier in this process")
+    assert lib.hycom_hybgen_batch_tuning(None, None, None, None, 8) == 0
+    assert lib.hycom_hybgen_batch_tuning(None, None, None, None, -1) == abi.HYB_EINVAL
+    lv, km = ctypes.c_int32(-5), ctypes.c_int32(-5)
+    assert lib.hycom_hybgen_batch_staged_levels(ctypes.byref(lv), ctypes.byref(km)) == 0
+    assert lv.value == 0 and km.value == 0

@@ -66,8 +66,11 @@ def test_batch_matches_single_domains(gpu):
     abi.check(abi.load().hycom_hybgen_batch_autotune(1))
     pairs = _pairs(RAGGED)
     batch = abi.Batch([a for _, _, a, _ in pairs])
-    # 20 calls: every candidate width pair twice, then the tuned one
-    for call in range(20):
+    # every candidate shape twice (remap columns x staged levels, then the
+    # regrid widths), then the tuned one
+    for call in range(40):
+        if call > 0 and batch.config()[2]:
+            break
         batch.hybgen(m=1, n=2)
         for _, _, _, b in pairs:
             b.hybgen(m=1, n=2)
@@ -85,8 +88,11 @@ def test_batch_matches_single_domains(gpu):
                 df = ol.diff_domains(want, a.download())
                 assert not df, f"domain {spec} differs from the oracle: {df}"
     rc, gc, tuned = batch.config()
-    print(f"[tuned] remap {rc} regrid {gc} tuned={tuned}", flush=True)
+    ns, kmax = batch.staged_levels()
+    print(f"[tuned] remap {rc} regrid {gc}, {ns} of {kmax - 1} levels staged, tuned={tuned} "
+          f"after {call} calls", flush=True)
     assert tuned and rc in (64, 128, 256) and gc in (64, 128, 256)
+    assert ns in (kmax - 1, 24, 12) and 8 * ns * rc <= 65536
 
 
 def test_batch_both_time_levels_and_set_change(gpu):
