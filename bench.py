@@ -307,7 +307,8 @@ def trc_leg(lib, dd, P, sea, args, stream):
     bpc = bytes_per_column_trc(K, K, T)
     out = {"value": sea * K / (ms * 1e-3), "unit": "column-layers/s", "ms_per_call": ms,
            "method": "plmtrc", "ki": K, "ko": K, "ks": T, "bytes_per_sea_column": bpc,
-           "achieved_GBs": sea * bpc / (ms * 1e-3) / 1e9, "kernels": "k_trc"}
+           "achieved_GBs": sea * bpc / (ms * 1e-3) / 1e9,
+           "kernels": "k_trc<1,4,3> (plmtrc, 4 fields per scan, slopes formed as the scan goes)"}
     print(f"[bench] trc {ms:.2f} ms/call", file=sys.stderr, flush=True)
     del si, so, pi
     torch.cuda.empty_cache()
@@ -777,6 +778,10 @@ def main():
             traffic = tj.get("bytes_per_launch_by_kernel", {}).get(kname.split(" ")[0])
         else:
             traffic = tj.get("bytes_per_launch")
+        # HBM bytes per launch of the adjacent remaps' kernels (same profile)
+        for leg, kn in ((stokes, "k_stokes_walk"), (trc, "k_trc<1,4,3>")):
+            if leg is not None and kn in tj.get("bytes_per_launch_by_kernel", {}):
+                leg["traffic_bytes_per_launch"] = tj["bytes_per_launch_by_kernel"][kn]
 
     cpu = None
     if not args.no_cpu_baseline:
