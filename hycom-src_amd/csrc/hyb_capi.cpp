@@ -92,7 +92,7 @@ int batch_domain_ok(const hyb_params* P, const hyb_fields* F, int d) {
   if ((rc = hycom_hybgen_check_params(P))) return rc;
   if ((rc = check_fields(P, F, 0))) return rc;
   if (!stream_path_ok(kparams_of(P)))
-    return fail(HYB_EUNSUPPORTED, "domain %d: batches need nhybrd==kdm, no HYBGEN_DENFIX", d);
+    return fail(HYB_EUNSUPPORTED, "domain %d: batches need nhybrd==kdm", d);
   if (P->itracr801 || P->itracr802 || P->itracr803)
     return fail(HYB_EUNSUPPORTED, "domain %d: no itracr diagnostics in batches", d);
   return 0;
@@ -748,7 +748,7 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   if (m < 1 || m > 2 || n < 1 || n > 2 || (raflag && m == n))
     return fail(HYB_EINVAL, "time levels m=%d n=%d", m, n);
   if (P->precision == HYB_PREC_MIXED && !stream_path_ok(kparams_of(P)))
-    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm and no HYBGEN_DENFIX");
+    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm");
   if (P->precision == HYB_PREC_MIXED && (g_kernel_path == 1 || g_kernel_path == 2))
     return fail(HYB_EUNSUPPORTED, "the mixed form runs on the streaming kernels only");
   if ((rc = device_ready())) return rc;
@@ -759,7 +759,7 @@ int hycom_hybgen_device(const hyb_params* P, const hyb_fields* F, int m, int n, 
   V.rec_kp = g_rec_kp;
   const int path = g_kernel_path;
   if (path == 3 && !stream_path_ok(K))
-    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
+    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm");
   const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
   if (use_stream) {
     Scratch W;
@@ -839,13 +839,13 @@ int hycom_hybgen_device_noalloc(const hyb_params* P, const hyb_fields* F, int m,
     return fail(HYB_EUNSUPPORTED,
                 "check_finite needs a result buffer and a host wait; use hycom_hybgen_device");
   if (P->precision == HYB_PREC_MIXED && !stream_path_ok(kparams_of(P)))
-    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm and no HYBGEN_DENFIX");
+    return fail(HYB_EUNSUPPORTED, "the mixed form needs nhybrd==kdm");
   if (P->precision == HYB_PREC_MIXED && g_kernel_path == 1)
     return fail(HYB_EUNSUPPORTED, "the mixed form runs on the streaming kernels only");
   KParams K = kparams_of(P);
   const int path = g_kernel_path;
   if (path == 3 && !stream_path_ok(K))
-    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm and no HYBGEN_DENFIX");
+    return fail(HYB_EUNSUPPORTED, "streaming kernels need nhybrd==kdm");
   const bool use_stream = (path == 0 || path == 3) && stream_path_ok(K);
   const int64_t nij = shape_of(P).NIJ;
   const size_t need = noalloc_need(nij, P->kdm, raflag != 0);

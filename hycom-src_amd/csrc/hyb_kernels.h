@@ -21,10 +21,10 @@ struct PhaseTimer {
 hipError_t launch_hybgen_fp64(const KParams& P, const View& V, int m, int n,
                               int raflag, hipStream_t s, PhaseTimer* T = nullptr);
 
-// True when the streaming kernels cover this parameter pack (any kdm,
-// nhybrd==kdm, no HYBGEN_DENFIX).
+// True when the streaming kernels cover this parameter pack (any kdm <= 64
+// with nhybrd==kdm; HYBGEN_DENFIX and the RA branch included).
 bool stream_path_ok(const KParams& P);
-// remapper of the u/v phase: 0 PCM, 1 PLM, 3 PPM (hybgen.F90:1960-1975)
+// remapper of the u/v phase: 0 PCM, 1 PLM, 3 WENO (hybgen.F90:1811-1824)
 int uv_method_of(const KParams& P);
 // Kernel variants of the streaming family (measurement and tuning):
 // remap_variant picks the p-column remap kernel (hyb_k_remap.inc),

@@ -588,7 +588,7 @@ int hycom_hybgen_server_add_domain(hyb_server* s, const hyb_params* P, const hyb
   d->k[0] = kparams_of(&d->P);
   d->k[1] = kparams_of(&pm);
   if (!stream_path_ok(d->k[0]))
-    return fail(HYB_EUNSUPPORTED, "served domains need nhybrd==kdm, no HYBGEN_DENFIX");
+    return fail(HYB_EUNSUPPORTED, "served domains need nhybrd==kdm");
   if (P->itracr801 || P->itracr802 || P->itracr803)
     return fail(HYB_EUNSUPPORTED, "no itracr diagnostics in served domains");
   d->L = layout_of(d->P);
