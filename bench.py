@@ -120,6 +120,29 @@ def cpu_baseline(workload, hybmap, rows, seed):
             lib.omp_set_num_threads(nmax)
         one = {"value": sea1 * P1.kdm / t1, "rows": P1.jdm, "sea_columns": sea1,
                "seconds_per_call": t1}
+    # the same reference on the whole config-1 domain (150x100x32, PPM), at
+    # the same thread count and on one thread
+    cfg1 = None
+    try:
+        Pc, wc = abi.workload_params("cfg1", hybmap=hybmap)
+        dc = abi.HostDomain(Pc).synth(abi.make_synth(Pc, wc, seed=seed))
+        seac = int((dc.a["ip"][Pc.nbdy:Pc.nbdy + Pc.ii, Pc.nbdy:Pc.nbdy + Pc.jj] > 0).sum())
+        if has_omp:
+            lib.omp_set_num_threads(cores)
+        tc = sorted(ol.run_ref(dc.copy()) for _ in range(5))[2]
+        tc1 = None
+        if has_omp:
+            lib.omp_set_num_threads(1)
+            try:
+                tc1 = sorted(ol.run_ref(dc.copy()) for _ in range(5))[2]
+            finally:
+                lib.omp_set_num_threads(nmax)
+        cfg1 = {"workload": f"cfg1-{Pc.idm}x{Pc.jdm}x{Pc.kdm}", "sea_columns": seac,
+                "value": seac * Pc.kdm / tc, "threads": threads, "seconds_per_call": tc,
+                "one_thread_value": (seac * Pc.kdm / tc1) if tc1 else None,
+                "about": "median of 5 calls of the whole domain"}
+    except Exception as e:  # the band figure above stands on its own
+        cfg1 = {"error": str(e)}
     return {
         "value": sea * P.kdm / t,
         "unit": "column-layers/s",
@@ -131,6 +154,7 @@ def cpu_baseline(workload, hybmap, rows, seed):
                    f"calls of {t:.3f} s, {threads} OpenMP threads = physical cores of the "
                    f"{logical} CPUs this process may use (lscpu)"),
         "one_thread": one,
+        "config1": cfg1,
     }
 
 
@@ -769,7 +793,7 @@ def main():
     if not args.no_serve:
         serve = serve_leg(args)
 
-    traffic = None
+    traffic = valu = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_hybmap{args.hybmap}.json")
     if os.path.exists(tf):
         with open(tf) as f:
@@ -778,6 +802,7 @@ def main():
             traffic = tj.get("bytes_per_launch_by_kernel", {}).get(kname.split(" ")[0])
         else:
             traffic = tj.get("bytes_per_launch")
+        valu = tj.get("valu_instructions_per_launch", {}).get(kname.split(" ")[0])
         # HBM bytes per launch of the adjacent remaps' kernels (same profile)
         for leg, kn in ((stokes, "k_stokes_walk"), (trc, "k_trc<1,4,3>")):
             if leg is not None and kn in tj.get("bytes_per_launch_by_kernel", {}):
@@ -825,6 +850,11 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": sea * bpc_a,
             "bytes_per_sea_column": bpc_a,
+            # VALU issue time of the kernel's SQ_INSTS_VALU (profiled run of the
+            # same build) over its measured time: 4 cycles per wave64
+            # instruction on 1024 SIMDs at 2.4 GHz
+            "valu_issue_fraction": ((valu * 4 / (1024 * 2.4e9) * 1e3) / ms_a
+                                    if valu and ms_a else None),
         },
         "call_bytes_per_sea_column": bytes_per_column_call(P.kdm, P.ntracr),
         "call_effective_GBs": sea * bytes_per_column_call(P.kdm, P.ntracr) / (ms_step * 1e-3) / 1e9,
