@@ -58,3 +58,46 @@ def test_trc_out_of_order_interfaces_are_reported(gpu):
     assert rc == abi.HYB_EINVAL
     assert b"bad lf" in abi.load().hycom_hybgen_last_error()
     assert np.array_equal(a["so"], b["so"], equal_nan=True)
+
+
+def test_trc_glbb_window_matches_oracle(gpu):
+    """plmtrc as thermf calls it, on the inputs the bench's trc leg builds,
+    at window size: a 600x400 window of the GLBb field after one hybgen
+    step, 4 tracers of seeded climatology on interfaces half a layer off the
+    model's p, remapped onto p; against the oracle, bitwise."""
+    import torch
+    print("\n[case] trc GLBb window 600x400x41 after a hybgen step, plmtrc, 4 tracers",
+          flush=True)
+    P, w = abi.workload_params("glbb", idm=600, jdm=400)
+    S = abi.make_synth(P, w, seed=200, itdm=4500, jtdm=3298, i0=1800, j0=1200)
+    dd = abi.DeviceDomain(P, skip=("dpo",)).synth(S)
+    dd.hybgen(m=1, n=2)
+    torch.cuda.synchronize()
+    NI, NJ = P.idm + 2 * P.nbdy, P.jdm + 2 * P.nbdy
+    nij, K, T = NI * NJ, P.kdm, 4
+    p = dd.t["p"].view(K + 1, nij)
+    pi = torch.empty_like(p)
+    pi[0] = 0.0
+    pi[1:K] = 0.5 * (p[1:K] + p[2:K + 1])
+    pi[K] = p[K]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    si = torch.cumsum(torch.randn(T, K, nij, device="cuda", generator=g, dtype=torch.float64),
+                      dim=1) + 5.0
+    so = torch.zeros(T, K, nij, device="cuda", dtype=torch.float64)
+    TP = abi.HybTrcParams(idm=P.idm, jdm=P.jdm, ii=P.ii, jj=P.jj, nbdy=P.nbdy, ki=K, ko=K,
+                          ks=T, method=1, reserved=0)
+    F = abi.HybTrcFields()
+    F.si, F.pi, F.po = si.data_ptr(), pi.data_ptr(), p.data_ptr()
+    F.mask, F.so = dd.t["ip"].data_ptr(), so.data_ptr()
+    abi.check(abi.load().hycom_trc_remap_device(ctypes.byref(TP), ctypes.byref(F), None))
+    torch.cuda.synchronize()
+    got = so.cpu().numpy()
+    h = {"si": si.cpu().numpy(), "pi": pi.cpu().numpy(), "po": p.contiguous().cpu().numpy(),
+         "mask": dd.t["ip"].cpu().numpy(), "so": np.zeros((T, K, nij))}
+    H = abi.HybTrcFields()
+    for k, v in h.items():
+        setattr(H, k, v.ctypes.data)
+    assert ol.oracle().oracle_trc_remap(ctypes.byref(TP), ctypes.byref(H)) == 0
+    sea = h["mask"].reshape(nij) != 0
+    assert sea.sum() > 100000
+    assert np.array_equal(h["so"][:, :, sea], got[:, :, sea])
