@@ -29,6 +29,10 @@ struct PhaseMark {
   }
 };
 
+// hyb_k_uv.hip: the streaming u/v kernels of time level nl, then the
+// fallback points they flagged
+void launch_uv_stream(const KParams& P, const View& V, const Scratch& W, int nl,
+                      const StreamTuning& tune, hipStream_t s);
 // hyb_k_misc.hip
 hipError_t launch_halo(const KParams& P, const View& V, hipStream_t s);
 void launch_diag_pre(const KParams& P, const View& V, int n, hipStream_t s);
