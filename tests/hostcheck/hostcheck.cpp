@@ -82,6 +82,7 @@ extern "C" int hostcheck_hybgen(const hyb_params* P, const hyb_fields* F, int m,
 
 static int64_t g_unsafe_cols = 0;
 static int g_uv_recompute = 0;  // 1: the read-ahead regrid, remap and u/v variants
+static int g_split_ns = 7;      // interface levels the split remap stages (the rest from p)
 
 static int64_t g_uv_fallback = 0;
 static int64_t g_ra_fallback = 0;
@@ -131,9 +132,9 @@ static void run_stream(const KParams& P, const View& V, int m, int n, int raflag
         }
         continue;
       }
-      if (g_uv_recompute) {  // 7 interface levels staged, the rest read from p
+      if (g_uv_recompute) {  // g_split_ns interface levels staged, the rest read from p
         double srow[kKMax + 2];
-        SplitInterfaces pa = stage_split(srow, 1, V.p + c0, V.NIJ, kk, 7);
+        SplitInterfaces pa = stage_split(srow, 1, V.p + c0, V.NIJ, kk, g_split_ns);
         switch (P.hybmap) {
           case 0: remap_stream<0, 2, 0>(P, V, W, i, j, n, pa); break;
           case 1: remap_stream<1, 2, 0>(P, V, W, i, j, n, pa); break;
@@ -244,6 +245,9 @@ extern "C" int hostcheck_hybgen_stream_rc(const hyb_params* P, const hyb_fields*
   g_uv_recompute = 0;
   return rc;
 }
+
+// staged levels of the split remap in hostcheck_hybgen_stream_rc (>= 1)
+extern "C" void hostcheck_set_split_levels(int ns) { g_split_ns = ns; }
 
 extern "C" int64_t hostcheck_stream_unsafe_columns(void) { return g_unsafe_cols; }
 

@@ -79,3 +79,25 @@ def test_streaming_uv_fallback_points_are_exercised():
         found += hc.uv_fallback_points()
         assert not ol.diff_domains(a, b)
     assert found > 0
+
+
+@pytest.mark.parametrize("ns", [1, 2, 15, 39, 40, 64])
+@pytest.mark.parametrize("hybmap", [2, 3])
+def test_split_interface_staging(ns, hybmap):
+    """The remap with only ns interface levels staged (the batch kernels'
+    LDS split, SplitInterfaces: the deeper levels read from p as the walk
+    reaches them, starting again for each walk over the column), against
+    the oracle on a kdm-41 domain: from one staged level to all of them."""
+    P, w = abi.workload_params("cfg1", idm=40, jdm=30, kdm=41, hybmap=hybmap, ntracr=4,
+                               trcflg=[2, 0, 0, 0])
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=ns + 10 * hybmap))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, threads=4)
+    lib = hc.lib()
+    lib.hostcheck_set_split_levels(ns)
+    try:
+        hc.run("streaming-recompute", b)
+    finally:
+        lib.hostcheck_set_split_levels(7)
+    df = ol.diff_domains(a, b)
+    assert not df, f"ns={ns} hybmap={hybmap}: {df}"
