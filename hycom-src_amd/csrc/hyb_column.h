@@ -394,7 +394,7 @@ HYB_HD void column_a(const KParams& P, const View& V, int i, int j, int n,
   const double thkbop = P.thkbop;
   const double hybrlx = P.hybrlx;
   const double rthin = P.hybthn;
-  double q, qdep, qts, p_hat, p_hat0, p_hat2, p_hat3;
+  double q, qdep, qts, p_hat, p_hat0 = 0.0, p_hat2, p_hat3;
   int k, ka, kp, fixlay, fixall;
 
   uint64_t lcm = lcm_row;
@@ -720,8 +720,9 @@ HYB_HD void column_a(const KParams& P, const View& V, int i, int j, int n,
           ltop = FK(theta, k - 1) - FK(th3d, k - 1) >= FK(th3d, k) - FK(theta, k);
         }
         if (FK(th3d, k - 1) >= FK(theta, k - 1) || !ltop || p[k] <= dp0cum[k] + kOnem) {
-          p_hat0 = 0.0;
           if ((FK(theta, k) - FK(th3d, k - 1)) <= kEpsil) {
+            // p_hat0 keeps the previous level's 1479 value: the blocking
+            // branch below reads it (1301-1330)
             p_hat = p[k - 1] + dp0ij[k - 1];
           } else {
             q = (FK(theta, k) - FK(th3d, k)) / (FK(theta, k) - FK(th3d, k - 1));

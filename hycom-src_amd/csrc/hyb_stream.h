@@ -392,7 +392,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
 
   // ------------------------------------------------ pass 3: 1036-1529
   const double floor1 = 0.0 + dp0ij1;              // 1053-1055: p(1)+dp0ij(1)
-  const double pbot = zx < floor1 ? floor1 : zx;   // p(kk+1) after 1056-1062
+  // p(kk+1) after 1056-1062; the thick-thin branches below move it where
+  // the deepest layer starts with a negative thickness
+  double pbot = zx < floor1 ? floor1 : zx;
   double floor2 = 0.0;                             // 1098-1107, once set
   int floor2_from = K + 2;
   auto with_floor = [&](int x, double v) -> double {
@@ -513,7 +515,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
       nta3[q] = theta[clk(5 + q)];
     }
   }
-  double q, p_hat, p_hat0, p_hat2, p_hat3;
+  double q, p_hat, p_hat0 = 0.0, p_hat2, p_hat3;
 #pragma unroll 1
   for (int k = 2; k <= K; k++) {
     // pres(k-5) for the in-place check of consume(k-2), issued early
@@ -548,9 +550,14 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
         dp_bot = PW(k + 2) - PW(k + 1) + kEpsil;
         dp_far = PW(k + 3) - PW(k + 2) + kEpsil;
       }
-      // hybthk <= 0.9 makes the first test false at k == kk and the second
-      // false at k >= kk-1; the guards keep th3d(kk+1..) out of reach.
-      if (k < K && dp_cen < P.hybthk * dmin(dp_top, dp_bot)) {
+      // With thicknesses >= 0, hybthk <= 0.9 makes the first test false at
+      // k == kk and the second false at k >= kk-1.  A deepest layer of
+      // negative thickness (an interface under the bottom by a float32
+      // rounding, as a start from a climatology gives) makes the second
+      // true at kk-1 and the first true at kk; both then take the
+      // thickness weighting (min < 0.2*max), so th3d(kk+1) stays unread,
+      // and both move p(kk+1).
+      if (dp_cen < P.hybthk * dmin(dp_top, dp_bot)) {
         // 1131-1159 thick-thin-thick
         const double thp = thw[2], tkp = tw[2];
         if (dmin(dp_top, dp_bot) < 0.2 * dmax(dp_top, dp_bot)) {
@@ -569,7 +576,8 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
         PW(k) = (1.0 - q0) * PW(k) + q0 * p_hat;
         p_hat = PW(k + 1) + (1.0 - q) * dp_inc;
         PW(k + 1) = (1.0 - q0) * PW(k + 1) + q0 * p_hat;
-      } else if (k < K - 1 && dp_cen + dp_bot < P.hybthk * dmin(dp_top, dp_far)) {
+        if (k == K) pbot = PW(k + 1);
+      } else if (k < K && dp_cen + dp_bot < P.hybthk * dmin(dp_top, dp_far)) {
         // 1178-1208 thick-thin-thin-thick
         const double thp = thw[2], thp2 = thw[3], tkp2 = tw[3];
         const double thmid = 0.5 * (thk + thp);
@@ -589,6 +597,7 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
         PW(k) = (1.0 - q0) * PW(k) + q0 * p_hat;
         p_hat = PW(k + 2) + (1.0 - q) * dp_inc;
         PW(k + 2) = (1.0 - q0) * PW(k + 2) + q0 * p_hat;
+        if (k == K - 1) pbot = PW(k + 2);
       } else if (tk > thk + kEpsil && k > fixlay + 1) {
         // 1237-1387 layer too dense: move the upper interface up
         bool ltop;
@@ -600,8 +609,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
           ltop = thm - tkm >= tk - thk;
         }
         if (tkm >= thm || !ltop || PW(k) <= dp0cum_at(k, k) + kOnem) {
-          p_hat0 = 0.0;
           if ((thk - tkm) <= kEpsil) {
+            // p_hat0 keeps the previous level's 1479 value: the blocking
+            // branch below reads it (1301-1330)
             p_hat = PW(k - 1) + dm1;
           } else {
             q = (thk - tk) / (thk - tkm);

@@ -785,7 +785,7 @@ static void hybgenaj(const ctx_t* c, int m, int n, int j) {
 
   for (int i = 1; i <= c->P->ii; i++) {
     if (P2(F->ip, i, j) == 0) continue;
-    double q, qdep, qts, p_hat, p_hat0, p_hat2, p_hat3;
+    double q, qdep, qts, p_hat, p_hat0 = 0.0, p_hat2, p_hat3;
     int k, ka, kp, fixlay, fixall;
 
     /* terrain following, 399-477 */
@@ -1148,8 +1148,9 @@ static void hybgenaj(const ctx_t* c, int m, int n, int j) {
           if (F4(th3d, i, j, k - 1, n) >= P3(theta, i, j, k - 1) || !ltop ||
               P3(p, i, j, k) <= dp0cum[k] + ONEM) {
             if ((P3(theta, i, j, k) - F4(th3d, i, j, k - 1, n)) <= EPSIL) {
+              /* p_hat0 keeps the value the tail of the previous k set
+               * (1479); the blocking branch below reads it (1301-1330) */
               p_hat = P3(p, i, j, k - 1) + dp0ij[k - 1];
-              p_hat0 = 0.0; /* not read on this path */
             } else {
               q = (P3(theta, i, j, k) - F4(th3d, i, j, k, n)) /
                   (P3(theta, i, j, k) - F4(th3d, i, j, k - 1, n));

@@ -181,3 +181,48 @@ def trc_ref():
                            _P(ctypes.c_int)]
         _trc_ref = lib
     return _trc_ref
+
+
+def zlevel_start(dom, m=1, n=2, seed=0):
+    """Overwrites time levels m and n of a synthesized domain with a start
+    from a z-level climatology, the way the model starts with iniflg=2
+    (inicon.F90:70-92, 311-320): layer k spans fixed depths, interfaces
+    below the bottom sit on it, T and S follow smooth profiles and th3d is
+    their density.  The interfaces are float32 (the climatology's .a
+    records) and the bottom is depths*onem in double, so the deepest
+    layer's thickness is a float32 rounding, of either sign, where the
+    column is shallower than the last interface.  The first hybgen call
+    then regrids every column hard: too-dense layers take whole layers
+    from above, massless layers fill from above, interfaces block."""
+    import numpy as np
+    P = dom.P
+    rng = np.random.default_rng(seed)
+    nb, K = P.nbdy, P.kdm
+    onem = 9.806 * 1000.0
+    zi = np.concatenate([[0.0], np.cumsum(5.0 * 1.37 ** np.arange(K - 1))])
+    sig = oracle().oracle_sig
+    a = dom.a
+    for jj in range(P.jj):
+        for ii in range(P.ii):
+            i, j = nb + ii, nb + jj
+            if a["ip"][i, j] == 0:
+                continue
+            depth = a["depths"][i, j]
+            pbot = depth * onem
+            pw = np.array([float(np.float32(min(z, depth) * onem)) for z in zi])
+            dp = np.diff(np.concatenate([pw, [pbot]]))
+            sst = 26.0 - 4.0 * jj / max(P.jj, 1) + 0.3 * rng.standard_normal()
+            zm = [0.5 * (zi[k] + min(zi[k + 1] if k + 1 < K else 1e5, zi[k] + 400.0))
+                  for k in range(K)]
+            for lv in {m, n}:
+                for k in range(K):
+                    t = 4.0 + (sst - 4.0) * np.exp(-zm[k] / 500.0) - 2.0 * zm[k] / 5000.0
+                    s = 34.9 + 0.6 * np.exp(-zm[k] / 250.0) + 0.02 * rng.standard_normal()
+                    a["dp"][i, j, k, lv - 1] = dp[k]
+                    a["temp"][i, j, k, lv - 1] = t
+                    a["saln"][i, j, k, lv - 1] = s
+                    a["th3d"][i, j, k, lv - 1] = sig(t, s) - P.thbase
+            a["p"][i, j, 0] = 0.0
+            for k in range(K):
+                a["p"][i, j, k + 1] = a["p"][i, j, k] + dp[k]
+    return dom

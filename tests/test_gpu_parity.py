@@ -447,3 +447,26 @@ def test_decisions_match_oracle(gpu, kernel_path, case):
     # the decisions vary over the domain, so equality is not vacuous
     assert len(np.unique(fx_o[sea])) > 1 and len(np.unique(kp_o[sea])) > 1
     _assert_same(want, dd.download(), "outputs beside the decisions")
+
+
+@pytest.mark.parametrize("hybmap,raflag,hybthk", [(0, 0, 0.0), (2, 0, 0.0), (3, 1, 0.0),
+                                                  (1, 0, 0.9)])
+def test_zlevel_start_matches_oracle(gpu, kernel_path, hybmap, raflag, hybthk):
+    """A start from a z-level climatology (oracle_lib.zlevel_start): whole
+    layers taken into too-dense massless ones, blocking layers, and
+    deepest layers of negative thickness."""
+    P, w = _case("zlevel-start", idm=40, jdm=30, kdm=16, hybmap=hybmap, hybthk=hybthk,
+                 hybrlx=8.0, ntracr=2, trcflg=[0, 2])
+    _select(kernel_path, P)
+    for seed, (m, n) in ((3, (1, 2)), (4, (2, 1))):
+        print(f"[seed] {seed} m,n={m},{n} raflag={raflag}", flush=True)
+        d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=seed), m=m, n=n)
+        ol.zlevel_start(d0, m=m, n=n, seed=seed)
+        if raflag:
+            d0.a["dpo"][...] = d0.a["dp"]
+        want = _oracle_out(d0, m=m, n=n, raflag=raflag)
+        dd = abi.DeviceDomain(P).upload(d0)
+        dd.hybgen(m=m, n=n, raflag=raflag)
+        import torch
+        torch.cuda.synchronize()
+        _assert_same(want, dd.download(), f"zlevel-start hybmap={hybmap} seed={seed}")

@@ -101,3 +101,26 @@ def test_split_interface_staging(ns, hybmap):
         lib.hostcheck_set_split_levels(7)
     df = ol.diff_domains(a, b)
     assert not df, f"ns={ns} hybmap={hybmap}: {df}"
+
+
+@pytest.mark.parametrize("family", ["streaming", "streaming-recompute", "lane-arrays"])
+@pytest.mark.parametrize("hybmap,raflag,hybthk", [(0, 0, 0.0), (2, 0, 0.0), (3, 0, 0.5),
+                                                  (2, 1, 0.0), (1, 1, 0.9)])
+def test_device_code_on_host_zlevel_start(family, hybmap, raflag, hybthk):
+    """The start from a z-level climatology (oracle_lib.zlevel_start): a
+    too-dense massless layer takes a whole layer from above and the
+    blocking-layer branch reads the previous level's p_hat0; a deepest
+    layer of negative thickness takes the thick-thin branches at kk-1 and
+    kk, which move the bottom interface."""
+    P, w = abi.workload_params("cfg1", idm=40, jdm=30, kdm=16, hybmap=hybmap, hybthk=hybthk,
+                               hybrlx=8.0, ntracr=2, trcflg=[0, 2])
+    for seed, (m, n) in ((3, (1, 2)), (4, (2, 1))):
+        d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=seed), m=m, n=n)
+        ol.zlevel_start(d, m=m, n=n, seed=seed)
+        if raflag:
+            d.a["dpo"][...] = d.a["dp"]
+        a, b = d.copy(), d.copy()
+        ol.run_oracle(a, m=m, n=n, raflag=raflag, threads=4)
+        hc.run(family, b, m=m, n=n, raflag=raflag)
+        df = ol.diff_domains(a, b)
+        assert not df, f"{family} hybmap={hybmap} raflag={raflag} seed={seed}: {df}"

@@ -1,0 +1,164 @@
+"""The stand-alone HYCOM model, stock against the drop-in.
+
+oracle/ref/Makefile `model` builds the model from the HYCOM-src sources
+twice: `hycom` as it lies, and `hycom_hip`, the same objects with the
+Fortran shim's `hybgen` (hycom-src_amd/fortran/hybgen_hip.F90 over
+libhycom_hybgen.so) in place of hybgen.F90's.  The two run side by side on
+one case (model_case.py) with the model's own comparator: PIPE_MASTER in
+one run directory and PIPE_SLAVE in the other name the same named pipe, and
+after every phase of every step (mod_hycom.F90:2514-2651, the
+`pipe_comparall` calls) the master compares every field of the slave with
+its own, bit for bit, and stops at the first difference (mod_pipe.F90:
+402-487).  The runs start from a z-level climatology, so hybgen regrids
+every column hard in the first steps, and through inicon's two calls
+(inicon.F90:544,559) before them.
+
+CPU tests: the case runs and two stock runs agree; a PLM run against a PPM
+run differs first after hybgen (the comparison sees hybgen's result).
+The shim over the oracle (hycom_oracle) against the stock model, on the CPU.
+GPU tests: stock against the drop-in, for each remap, the RA branch and
+tracers.  Every case prints its name before it starts."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import model_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "oracle", "_ref", "model")
+STOCK = os.path.join(MODEL, "hycom")
+DROPIN = os.path.join(MODEL, "hycom_hip")
+STEPS_PER_DAY = 96  # baclin 900 s
+
+needs_stock = pytest.mark.skipif(not os.path.exists(STOCK),
+                                 reason="oracle/_ref/model/hycom not built (make -C oracle/ref model)")
+
+
+def run_pair(tmp, master, slave, master_case, slave_case=None, timeout=90):
+    """Runs `master` and `slave` on their cases with a named pipe between
+    them; returns the master's comparison log and both run logs."""
+    m, s = os.path.join(tmp, "master"), os.path.join(tmp, "slave")
+    model_case.write_case(m, **master_case)
+    model_case.write_case(s, **(slave_case or master_case))
+    fifo = os.path.join(tmp, "pipe")
+    os.mkfifo(fifo)
+    for d, name in ((m, "PIPE_MASTER"), (s, "PIPE_SLAVE")):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(fifo + "\n")
+    logs = {}
+    procs = []
+    for d, exe in ((m, master), (s, slave)):
+        logs[d] = open(os.path.join(d, "run.log"), "w")
+        procs.append(subprocess.Popen([exe], cwd=d, stdout=logs[d], stderr=subprocess.STDOUT))
+    rcs = []
+    try:
+        for p in procs:
+            rcs.append(p.wait(timeout=timeout))
+    finally:
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+                p.wait()
+        for f in logs.values():
+            f.close()
+    read = lambda p: open(p).read() if os.path.exists(p) else ""  # noqa: E731
+    return (rcs, read(os.path.join(m, "PIPE_base.out")),
+            read(os.path.join(m, "run.log")), read(os.path.join(s, "run.log")))
+
+
+def ok_steps(base, phase):
+    return [int(x) for x in re.findall(rf"^{phase}, step\s+(\d+) -- OK", base, re.M)]
+
+
+def first_difference(base):
+    """The phase and field of the master's first reported difference, and
+    the lines reporting it."""
+    bad = re.search(r"^i,j=.*(error|slave):|ERROR|out of sync", base, re.M)
+    if not bad:
+        return None
+    head = base[:bad.start()]
+    phase = re.findall(r"^(\w+, step\s+\d+)$", head, re.M)
+    lines = base[bad.start():].splitlines()[:40]
+    return f"first difference after [{phase[-1] if phase else '?'}]:\n" + "\n".join(lines)
+
+
+def assert_identical(rcs, base, mlog, slog, days):
+    diff = first_difference(base)
+    assert diff is None, diff
+    assert rcs == [0, 0], f"exit codes {rcs}\n{mlog[-1500:]}\n{slog[-1500:]}"
+    assert "(normal)" in mlog and "(normal)" in slog
+    # inicon's two calls (inicon.F90:544-560), then one per time step
+    assert ok_steps(base, "inicn1") == [0] and ok_steps(base, "inicn2") == [0]
+    steps = ok_steps(base, "hybgen")
+    n = int(round(days * STEPS_PER_DAY))
+    assert len(steps) == n and steps[-1] - steps[0] == n - 1, steps[:5]
+    # the comparisons read every field: u, v, dp, T, S, th3d of each layer
+    assert len(re.findall(r"^reading for comparison: ", base, re.M)) > 100 * n
+
+
+@needs_stock
+def test_stock_against_stock(tmp_path):
+    print("\n[case] stock model master and slave, PPM, 1 day", flush=True)
+    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, STOCK, dict(hybmap=2, days=1.0))
+    assert_identical(rcs, base, mlog, slog, 1.0)
+    # the start regrids: the z-level climatology is far from the targets
+    assert "rdrlax" in mlog and "hybgen" in mlog
+
+
+@needs_stock
+def test_comparison_sees_a_hybgen_difference(tmp_path):
+    """PPM master, PLM slave: every phase before the first hybgen call
+    agrees, and the master stops at the comparison right after it (inicon's
+    first call, inicon.F90:544-545)."""
+    print("\n[case] stock PPM master against stock PLM slave", flush=True)
+    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, STOCK, dict(hybmap=2, days=0.25),
+                                     dict(hybmap=1, days=0.25), timeout=60)
+    assert "(pipe_compare_all)" in mlog, "the master did not stop at a difference"
+    bad = re.search(r"^i,j=.*(error|slave):", base, re.M)  # mod_pipe.F90:458-466
+    assert bad
+    first = bad.start()
+    phases = re.findall(r"^(\w+), step\s+(\d+)$", base[:first], re.M)
+    assert phases[-1] == ("inicn1", "0"), phases
+    assert all(re.search(rf"^{p}, step\s+{s} -- OK", base, re.M) for p, s in phases[:-1])
+
+
+ORACLE_DROPIN = os.path.join(MODEL, "hycom_oracle")
+
+CASES = [
+    ("pcm", dict(hybmap=0)),
+    ("plm", dict(hybmap=1)),
+    ("ppm", dict(hybmap=2)),
+    ("weno", dict(hybmap=3)),
+    ("ppm_ra", dict(hybmap=2, hybraf=True)),
+    ("ppm_tracers", dict(hybmap=2, ntracr=2, trcflg=23)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="oracle/_ref/model/hycom_hip not built (make -C oracle/ref model)")
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_stock_against_dropin(tmp_path, name, kw):
+    days = 1.0
+    print(f"\n[case] stock master (CPU) against the drop-in slave (GPU hybgen), {name}, "
+          f"{int(days * STEPS_PER_DAY)} steps", flush=True)
+    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, DROPIN, dict(kw, days=days))
+    assert_identical(rcs, base, mlog, slog, days)
+    if kw.get("ntracr"):
+        assert re.search(r"^reading for comparison: .*tracer|^reading for comparison: trc",
+                         base, re.M | re.I), "no tracer comparisons"
+
+
+@pytest.mark.skipif(not os.path.exists(ORACLE_DROPIN),
+                    reason="oracle/_ref/model/hycom_oracle not built (make -C oracle/ref model)")
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_stock_against_shim_over_oracle(tmp_path, name, kw):
+    """The shim over the oracle (oracle/ref/oracle_as_dropin.c) in place of
+    the GPU library, on the CPU: the oracle, and the shim's packing of the
+    model's state, inside the model's own time loop."""
+    days = 1.0
+    print(f"\n[case] stock master against the shim over the oracle, {name}", flush=True)
+    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, ORACLE_DROPIN, dict(kw, days=days))
+    assert_identical(rcs, base, mlog, slog, days)

@@ -134,3 +134,30 @@ def test_multi_step_oracle_matches_reference():
         ol.run_oracle(b, m=m, n=n, raflag=1)
         df = ol.diff_domains(a, b)
         assert not df, f"step {step}: {df}"
+
+
+@pytest.mark.parametrize("kdm", [16, 41])
+@pytest.mark.parametrize("hybmap", [0, 1, 2, 3])
+@pytest.mark.parametrize("raflag,ldenfix,hybthk", [(0, 0, 0.0), (1, 0, 0.0), (0, 1, 0.0),
+                                                   (0, 0, 0.5), (1, 0, 0.9)])
+def test_oracle_matches_reference_zlevel_start(kdm, hybmap, raflag, ldenfix, hybthk):
+    """A start from a z-level climatology (oracle_lib.zlevel_start): the
+    first call takes whole layers into too-dense massless ones and reaches
+    the blocking-layer branch right after (hybgen.F90:1259-1330), whose
+    p_hat0 is the previous level's value from 1479; a deepest layer of
+    negative thickness takes the thick-thin branches (1131-1208) at kk-1
+    and kk."""
+    rng = np.random.default_rng(700 + kdm + 10 * hybmap + raflag + 3 * ldenfix + int(10 * hybthk))
+    P, w = abi.workload_params("cfg1", idm=40, jdm=30, kdm=kdm, hybmap=hybmap, hybthk=hybthk,
+                               hybrlx=8.0, ldenfix=ldenfix, ntracr=2, trcflg=[0, 2])
+    for mn in ((1, 2), (2, 1)):
+        d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=int(rng.integers(1000))),
+                                    m=mn[0], n=mn[1])
+        ol.zlevel_start(d, m=mn[0], n=mn[1], seed=int(rng.integers(1000)))
+        if raflag:
+            d.a["dpo"][...] = d.a["dp"]
+        a, b = d.copy(), d.copy()
+        ol.run_oracle(a, m=mn[0], n=mn[1], raflag=raflag, threads=4)
+        ol.run_ref(b, m=mn[0], n=mn[1], raflag=raflag)
+        df = ol.diff_domains(a, b)
+        assert not df, f"kdm={kdm} hybmap={hybmap} raflag={raflag} m,n={mn}: {df}"
