@@ -179,11 +179,6 @@ struct Mirror {
   double *ubavg = nullptr, *vbavg = nullptr;
   int32_t *ip = nullptr, *iu = nullptr, *iv = nullptr;
   hipStream_t stream = nullptr;
-  // host arrays the cached device copies came from: the fields the call
-  // only reads and never changes (theta, depths, topiso, depthu, depthv,
-  // the masks), and p, pu, pv, whose land and halo values the call keeps
-  const void* host_const[8] = {};
-  const void* host_p[3] = {};
   int64_t up = 0, down = 0;  // bytes moved by the last call
   void release() {
     void* ptrs[] = {dp, dpo, dpu, dpv, temp, saln, th3d, u, v, tracer, p, pu, pv, theta,
@@ -956,12 +951,14 @@ int hycom_hybgen_timing_read(double* ms_sum, int32_t* count, int nphase) {
 
 // Copy mode moves what the call reads and writes (hybgen.F90:14-306 with
 // the fields of hybgenaj/bj/cj): time level n of dp, dpu, dpv, temp, saln,
-// th3d, u, v and the tracers both ways, level n of ubavg/vbavg up, and p,
-// pu, pv down; with hybgen_raflag also level m of those fields and dpo.
-// theta, depths, topiso, depthu, depthv and the masks go up once per host
-// array (hycom_hybgen_copy_invalidate() drops the cache).  p, pu and pv are
-// rewritten by the call at every sea point; their land and halo values go
-// up once per host array and come back unchanged with every download.
+// th3d, u, v and the tracers both ways, level n of ubavg/vbavg up, p, pu,
+// pv both ways, and theta, depths, topiso, depthu, depthv and the masks
+// up; with hybgen_raflag also level m of those fields and dpo.  Nothing is
+// taken as unchanged from one call to the next: the model changes some of
+// these in place between calls (dpthuv sets depthu/depthv after inicon's
+// two hybgen calls, mod_hycom.F90:1495,1684; its halo updates rewrite the
+// halo of p), and p, pu, pv come back whole, so their land and halo
+// values must be the ones the caller holds.
 int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raflag) {
   int rc = hycom_hybgen_check_params(P);
   if (rc) return rc;
@@ -1025,27 +1022,17 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
     }
     return e;
   };
-  // read-only fields, once per host array
-  const void* hc[8] = {F->theta, F->depths, F->topiso, F->depthu, F->depthv, F->ip, F->iu, F->iv};
-  if (memcmp(hc, M.host_const, sizeof hc) != 0) {
-    HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
-    HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
-    HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
-    HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
-    HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
-    HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
-    HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
-    HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
-    memcpy(M.host_const, hc, sizeof hc);
-  }
-  // p, pu, pv: land and halo values, once per host array
-  const void* hp[3] = {F->p, F->pu, F->pv};
-  if (memcmp(hp, M.host_p, sizeof hp) != 0) {
-    HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
-    HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
-    HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
-    memcpy(M.host_p, hp, sizeof hp);
-  }
+  HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
+  HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
+  HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
+  HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
+  HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
+  HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
+  HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
+  HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
+  HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
+  HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
+  HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
   HIPCHK(level(n, true));
   if (raflag) {
     HIPCHK(level(m, true));
@@ -1074,12 +1061,7 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
   return 0;
 }
 
-int hycom_hybgen_copy_invalidate(void) {
-  std::lock_guard<std::mutex> lk(g_mirror_mu);
-  memset(g_mirror.host_const, 0, sizeof g_mirror.host_const);
-  memset(g_mirror.host_p, 0, sizeof g_mirror.host_p);
-  return 0;
-}
+int hycom_hybgen_copy_invalidate(void) { return 0; }  // nothing is cached
 
 int hycom_hybgen_copy_stats(int64_t* bytes_up, int64_t* bytes_down) {
   std::lock_guard<std::mutex> lk(g_mirror_mu);

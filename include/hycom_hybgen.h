@@ -171,12 +171,12 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F,
 /* Copy mode moves only what the call reads and writes: time level n of
  * dp, dpu, dpv, temp, saln, th3d, u, v and the tracers both ways (and
  * level m plus dpo with hybgen_raflag), level n of ubavg/vbavg up, p, pu,
- * pv down.  theta, depths, topiso, depthu, depthv and the masks are
- * uploaded once per host array and taken as unchanged afterwards; so are
- * the land and halo values of p, pu, pv, which the call does not write.
- * A caller that changes any of those in place between calls calls
- * hycom_hybgen_copy_invalidate() first.  hycom_hybgen_copy_stats() gives
- * the bytes the last copy-mode call moved each way.  No reference
+ * pv both ways, and theta, depths, topiso, depthu, depthv and the masks
+ * up.  Every call moves all of it: the model changes some of these arrays
+ * in place between calls (depthu/depthv after inicon's calls, the halo of
+ * p).  hycom_hybgen_copy_invalidate() has nothing to drop and returns 0
+ * (kept for callers of the earlier cached form).  hycom_hybgen_copy_stats()
+ * gives the bytes the last copy-mode call moved each way.  No reference
  * counterpart (the reference has no device copies). */
 int hycom_hybgen_copy_invalidate(void);
 int hycom_hybgen_copy_stats(int64_t* bytes_up, int64_t* bytes_down);

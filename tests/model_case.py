@@ -114,9 +114,17 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
         "intf": [np.minimum(zi[k], depth) * ONEM for k in range(kdm)],
     }
     if ntracr:
-        clim["trcr"] = [(10.0 + 3.0 * q) * np.exp(-z / (150.0 + 200.0 * q))
+        # trcflg's decimal digits, least significant first, the last one
+        # repeated (blkdat.F90:1288-1300); a temperature tracer (2) starts
+        # from T's climatology, the others from their own profiles
+        flags, t = [], trcflg
+        for q in range(ntracr):
+            flags.append(t % 10)
+            t = t // 10 if t >= 10 else t
+        clim["trcr"] = [clim["temp"][k] if flags[q] == 2 else
+                        (10.0 + 3.0 * q) * np.exp(-zmid[k] / (150.0 + 200.0 * q))
                         * (1.0 + 0.3 * np.cos(0.2 * ii + q)) + 0.01 * (100.0 * yj + q)
-                        for q in range(ntracr) for z in zmid]
+                        for q in range(ntracr) for k in range(kdm)]
     for name, fl in clim.items():
         with open(P(f"relax.{name}.a"), "wb") as fa, open(P(f"relax.{name}.b"), "w") as fb:
             fb.write(f"z-level {name} climatology\n\n\n\n\n")

@@ -106,22 +106,28 @@ def test_copy_mode_matches_oracle(gpu):
 
 
 def _copy_bytes(P, raflag):
-    """Bytes copy mode moves per call once its caches hold (see
-    hycom_hybgen_copy_stats): levels n (and m, dpo with raflag)."""
+    """Bytes copy mode moves per call (see hycom_hybgen_copy_stats): levels
+    n (and m, dpo with raflag) of the 3-d fields both ways, ubavg/vbavg of
+    those levels up, p, pu, pv both ways, theta, depths, topiso, depthu,
+    depthv and the three masks up."""
     NIJ = (P.idm + 2 * P.nbdy) * (P.jdm + 2 * P.nbdy)
     lev = 8 * NIJ * P.kdm
     nl = 2 if raflag else 1
-    up = nl * ((8 + P.ntracr) * lev + 2 * 8 * NIJ) + (2 * lev if raflag else 0)
-    down = nl * (8 + P.ntracr) * lev + 3 * 8 * NIJ * (P.kdm + 1)
+    pp = 3 * 8 * NIJ * (P.kdm + 1)
+    const = lev + 4 * 8 * NIJ + 3 * 4 * NIJ
+    up = nl * ((8 + P.ntracr) * lev + 2 * 8 * NIJ) + (2 * lev if raflag else 0) + pp + const
+    down = nl * (8 + P.ntracr) * lev + pp
     return up, down
 
 
 def test_copy_mode_sequence_matches_oracle(gpu):
     """Successive copy-mode calls on one host domain, as a model makes
     them: time levels alternating, a Robert-Asselin call, a second domain
-    of the same shape (new host arrays), and a call after theta changed in
-    place with the cache dropped.  Every call against the oracle on the same
-    sequence, and the bytes each call moved."""
+    of the same shape (new host arrays), and calls after the caller changed
+    theta, depthu/depthv and the halo of p in place (the model sets
+    depthu/depthv after inicon's hybgen calls, mod_hycom.F90:1495,1684).
+    Every call against the oracle on the same sequence, and the bytes each
+    call moved."""
     lib = abi.load()
     P, w = _case("copy-sequence", idm=90, jdm=57, kdm=41, hybmap=2, ntracr=3,
                  trcflg=[2, 0, 0])
@@ -140,8 +146,7 @@ def test_copy_mode_sequence_matches_oracle(gpu):
         got.hybgen(m=m, n=n, raflag=ra)
         _assert_same(want, got, f"copy mode call {q}")
         abi.check(lib.hycom_hybgen_copy_stats(ctypes.byref(up), ctypes.byref(down)))
-        if q >= 1:  # caches filled by the first call
-            assert (up.value, down.value) == _copy_bytes(P, ra), (q, up.value, down.value)
+        assert (up.value, down.value) == _copy_bytes(P, ra), (q, up.value, down.value)
     # a second domain of the same shape: its own land values of p, pu, pv
     d1 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=32))
     for name in ("p", "pu", "pv"):
@@ -150,13 +155,22 @@ def test_copy_mode_sequence_matches_oracle(gpu):
     ol.run_oracle(w1, threads=8)
     g1.hybgen(m=1, n=2)
     _assert_same(w1, g1, "copy mode, second domain")
-    # theta changed in place: the caller drops the cache
+    # fields changed in place between calls, with nothing said to the library
     want.a["theta"][...] += 0.01
     got.a["theta"][...] += 0.01
-    abi.check(lib.hycom_hybgen_copy_invalidate())
     ol.run_oracle(want, m=1, n=2, threads=8)
     got.hybgen(m=1, n=2)
     _assert_same(want, got, "copy mode after theta changed")
+    for name in ("depthu", "depthv"):
+        want.a[name][...] *= 0.75
+        got.a[name][...] *= 0.75
+    nb = P.nbdy
+    for x in (want, got):
+        x.a["p"][:nb] = 123.0
+        x.a["p"][:, -nb:] = -7.0
+    ol.run_oracle(want, m=2, n=1, threads=8)
+    got.hybgen(m=2, n=1)
+    _assert_same(want, got, "copy mode after depthu/depthv and the halo of p changed")
 
 
 def test_device_synth_matches_host_synth(gpu):

@@ -146,9 +146,21 @@ def test_stock_against_dropin(tmp_path, name, kw):
           f"{int(days * STEPS_PER_DAY)} steps", flush=True)
     rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, DROPIN, dict(kw, days=days))
     assert_identical(rcs, base, mlog, slog, days)
-    if kw.get("ntracr"):
-        assert re.search(r"^reading for comparison: .*tracer|^reading for comparison: trc",
-                         base, re.M | re.I), "no tracer comparisons"
+    check_tracers_and_archives(str(tmp_path), kw)
+
+
+def check_tracers_and_archives(tmp, kw):
+    """The pipe compares u, v, dp, dpu, dpv, T, S and th3d, not the
+    tracers (mod_pipe.F90:1052-1700): the archives the two runs write
+    (every field, tracers included, as float32 .a records, mod_archiv.F90)
+    must be the same bytes."""
+    m, s = os.path.join(tmp, "master"), os.path.join(tmp, "slave")
+    arch = sorted(f for f in os.listdir(m) if f.startswith("archv.") and f.endswith(".a"))
+    assert len(arch) >= 2 and arch == sorted(
+        f for f in os.listdir(s) if f.startswith("archv.") and f.endswith(".a"))
+    for f in arch:
+        with open(os.path.join(m, f), "rb") as a, open(os.path.join(s, f), "rb") as b:
+            assert a.read() == b.read(), f"archive {f} differs"
 
 
 @pytest.mark.skipif(not os.path.exists(ORACLE_DROPIN),
@@ -162,3 +174,4 @@ def test_stock_against_shim_over_oracle(tmp_path, name, kw):
     print(f"\n[case] stock master against the shim over the oracle, {name}", flush=True)
     rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, ORACLE_DROPIN, dict(kw, days=days))
     assert_identical(rcs, base, mlog, slog, days)
+    check_tracers_and_archives(str(tmp_path), kw)
