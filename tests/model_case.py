@@ -57,7 +57,7 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
                trcflg=3, days=1.0):
     """Writes the case into directory `path` (created if missing).  The run
     starts at day 1 (step 96 with baclin 900 s) and ends `days` later."""
-    assert 4 <= kdm <= len(ZI) and idm >= 8 and jdm >= 8
+    assert 4 <= kdm <= 64 and idm >= 8 and jdm >= 8
     os.makedirs(path, exist_ok=True)
     P = lambda f: os.path.join(path, f)  # noqa: E731
 
@@ -103,7 +103,10 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
 
     # layer target densities (sigma-2) and the z-level climatology
     sigma = np.round(28.10 + (36.70 - 28.10) * np.linspace(0.0, 1.0, kdm) ** 0.8, 4)
-    zi = ZI[:kdm]
+    # z-levels: ZI, or for more layers thicknesses growing 13% a layer
+    # from 5 m (40 layers reach 5000 m)
+    zi = ZI[:kdm] if kdm <= len(ZI) else np.concatenate(
+        [[0.0], np.cumsum(5.0 * 1.13 ** np.arange(kdm - 1))])
     zmid = [0.5 * (zi[k] + min(zi[k + 1] if k + 1 < kdm else 1e5, zi[k] + 400.0))
             for k in range(kdm)]
     sst = 27.0 - 0.45 * (plat - 15.0) + 0.8 * np.sin(0.3 * plon)
@@ -140,7 +143,10 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     F = lambda n, v: L.append("%11s    '%-6s'" % (repr(float(v)), n))  # noqa: E731
     I("iversn", 23); I("iexpt", 10); I("idm", idm); I("jdm", jdm)
     I("itest", -1); I("jtest", -1); I("kdm", kdm); I("nhybrd", kdm); I("nsigma", 0)
-    F("dp00", 3.0); F("dp00x", 450.0); F("dp00f", 1.18)
+    # minimum thicknesses dp0k = dp00*dp00f**(k-1) up to dp00x; with many
+    # layers a smaller start and cap keep their sum a few hundred metres
+    dp00, dp00x = (3.0, 450.0) if kdm <= len(ZI) else (1.0, 12.0)
+    F("dp00", dp00); F("dp00x", dp00x); F("dp00f", 1.18)
     F("ds00", 1.0); F("ds00x", 15.0); F("ds00f", 1.18); F("dp00i", 1.0); F("isotop", 6.0)
     F("oneta0", 0.1); F("saln0", 35.0); I("locsig", 0); I("kapref", 0); I("thflag", 2)
     F("thbase", 34.0); I("vsigma", 0)

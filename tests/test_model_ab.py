@@ -133,6 +133,7 @@ CASES = [
     ("weno", dict(hybmap=3)),
     ("ppm_ra", dict(hybmap=2, hybraf=True)),
     ("ppm_tracers", dict(hybmap=2, ntracr=2, trcflg=23)),
+    ("weno_k41", dict(hybmap=3, kdm=41, idm=64, jdm=48)),
 ]
 
 
