@@ -54,9 +54,14 @@ def _recs(fa, fields, idm, jdm, mask=None):
 
 
 def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0,
-               trcflg=3, days=1.0):
+               trcflg=3, days=1.0, periodic=False, nsigma=0, hybthk=0.0, shelf=20.0):
     """Writes the case into directory `path` (created if missing).  The run
-    starts at day 1 (step 96 with baclin 900 s) and ends `days` later."""
+    starts at day 1 (step 96 with baclin 900 s) and ends `days` later.
+    periodic: a channel periodic in i (sea in the last column makes the
+    model take the domain as periodic, bigrid.F90:21-33), land rows at both
+    j edges; otherwise a closed basin.  shelf: the shallowest depth, m
+    (under the nsigma layers' deep limit dpns the columns are terrain
+    following, geopar.F90:449-630)."""
     assert 4 <= kdm <= 64 and idm >= 8 and jdm >= 8
     os.makedirs(path, exist_ok=True)
     P = lambda f: os.path.join(path, f)  # noqa: E731
@@ -84,13 +89,20 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
         for (name, _), (lo, hi) in zip(grid, rng):
             fb.write("%s: min,max = %16.8e %16.8e\n" % (name, lo, hi))
 
-    # a bowl of 4000 m with shelves down to 20 m, one land ring
+    # a bowl of 4000 m with shelves as shallow as `shelf`, one land ring; or a
+    # channel of the same section, periodic in i
     xi, yj = (ii + 0.5) / idm, (jj + 0.5) / jdm
-    r = np.maximum(np.abs(2 * xi - 1), np.abs(2 * yj - 1))
-    depth = 20.0 + 3980.0 * np.clip((1.0 - r) / 0.35, 0.0, 1.0) ** 1.5
-    depth += 150.0 * np.sin(7.0 * xi) * np.cos(5.0 * yj) * (depth > 400.0)
+    if periodic:
+        r = np.abs(2 * yj - 1)
+        bump = np.sin(2.0 * math.pi * 2.0 * xi) * np.cos(5.0 * yj)
+    else:
+        r = np.maximum(np.abs(2 * xi - 1), np.abs(2 * yj - 1))
+        bump = np.sin(7.0 * xi) * np.cos(5.0 * yj)
+    depth = shelf + (4000.0 - shelf) * np.clip((1.0 - r) / 0.35, 0.0, 1.0) ** 1.5
+    depth += 150.0 * bump * (depth > 400.0)
     depth[0, :] = depth[-1, :] = 0.0
-    depth[:, 0] = depth[:, -1] = 0.0
+    if not periodic:
+        depth[:, 0] = depth[:, -1] = 0.0
     sea = depth > 0.0
     with open(P("regional.depth.a"), "wb") as fa:
         (lo, hi), = _recs(fa, [depth], idm, jdm, mask=sea)
@@ -142,7 +154,7 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     I = lambda n, v: L.append("%11d    '%-6s'" % (int(v), n))  # noqa: E731,E741
     F = lambda n, v: L.append("%11s    '%-6s'" % (repr(float(v)), n))  # noqa: E731
     I("iversn", 23); I("iexpt", 10); I("idm", idm); I("jdm", jdm)
-    I("itest", -1); I("jtest", -1); I("kdm", kdm); I("nhybrd", kdm); I("nsigma", 0)
+    I("itest", -1); I("jtest", -1); I("kdm", kdm); I("nhybrd", kdm); I("nsigma", nsigma)
     # minimum thicknesses dp0k = dp00*dp00f**(k-1) up to dp00x; with many
     # layers a smaller start and cap keep their sum a few hundred metres
     dp00, dp00x = (3.0, 450.0) if kdm <= len(ZI) else (1.0, 12.0)
@@ -160,7 +172,7 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     I("stfflg", 0); F("stfrdt", 0.0); F("stfrds", 0.0); F("stfrdv", 0.0)
     F("ra2fac", 0.125); F("wbaro", 0.125); I("btrlfr", 1); I("btrmas", 0)
     I("hybraf", hybraf); F("hybrlx", 8.0); F("hybiso", 0.01); F("hybthn", 1.0)
-    F("hybthk", 0.0); I("hybmap", hybmap); I("hybflg", 0); I("advflg", 0)
+    F("hybthk", hybthk); I("hybmap", hybmap); I("hybflg", 0); I("advflg", 0)
     I("advtyp", 2); I("momtyp", 2); F("slip", 0.0); F("visco2", 0.05); F("visco4", 0.2)
     F("facdf4", 0.0); F("veldf2", 0.02); F("veldf4", 0.01); F("thkdf2", 0.0)
     F("thkdf4", 0.005); F("temdf2", 0.015); F("temdfc", 1.0); F("vertmx", 0.0)

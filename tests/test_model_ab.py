@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MODEL = os.path.join(ROOT, "oracle", "_ref", "model")
 STOCK = os.path.join(MODEL, "hycom")
 DROPIN = os.path.join(MODEL, "hycom_hip")
+DENFIX = os.path.join(MODEL, "denfix")  # the three builds with HYBGEN_DENFIX
 STEPS_PER_DAY = 96  # baclin 900 s
 
 needs_stock = pytest.mark.skipif(not os.path.exists(STOCK),
@@ -134,7 +135,22 @@ CASES = [
     ("ppm_ra", dict(hybmap=2, hybraf=True)),
     ("ppm_tracers", dict(hybmap=2, ntracr=2, trcflg=23)),
     ("weno_k41", dict(hybmap=3, kdm=41, idm=64, jdm=48)),
+    ("ppm_periodic", dict(hybmap=2, periodic=True)),
+    ("ppm_sigma_thk", dict(hybmap=2, nsigma=4, hybthk=0.5, shelf=8.0)),
+    ("plm_3days", dict(hybmap=1, days=3.0)),
+    ("ppm_denfix", dict(hybmap=2, denfix=True)),
 ]
+
+
+def builds(kw, dropin_name):
+    """The stock build and the named drop-in build for a case: the
+    HYBGEN_DENFIX builds where the case asks for them."""
+    d = DENFIX if kw.get("denfix") else MODEL
+    return os.path.join(d, "hycom"), os.path.join(d, dropin_name)
+
+
+def case_kw(kw):
+    return {k: v for k, v in kw.items() if k != "denfix"}
 
 
 @pytest.mark.gpu
@@ -142,10 +158,11 @@ CASES = [
                     reason="oracle/_ref/model/hycom_hip not built (make -C oracle/ref model)")
 @pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
 def test_stock_against_dropin(tmp_path, name, kw):
-    days = 1.0
+    days = kw.get("days", 1.0)
     print(f"\n[case] stock master (CPU) against the drop-in slave (GPU hybgen), {name}, "
           f"{int(days * STEPS_PER_DAY)} steps", flush=True)
-    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, DROPIN, dict(kw, days=days))
+    stock, dropin = builds(kw, "hycom_hip")
+    rcs, base, mlog, slog = run_pair(str(tmp_path), stock, dropin, dict(case_kw(kw), days=days))
     assert_identical(rcs, base, mlog, slog, days)
     check_tracers_and_archives(str(tmp_path), kw)
 
@@ -171,8 +188,9 @@ def test_stock_against_shim_over_oracle(tmp_path, name, kw):
     """The shim over the oracle (oracle/ref/oracle_as_dropin.c) in place of
     the GPU library, on the CPU: the oracle, and the shim's packing of the
     model's state, inside the model's own time loop."""
-    days = 1.0
+    days = kw.get("days", 1.0)
     print(f"\n[case] stock master against the shim over the oracle, {name}", flush=True)
-    rcs, base, mlog, slog = run_pair(str(tmp_path), STOCK, ORACLE_DROPIN, dict(kw, days=days))
+    stock, dropin = builds(kw, "hycom_oracle")
+    rcs, base, mlog, slog = run_pair(str(tmp_path), stock, dropin, dict(case_kw(kw), days=days))
     assert_identical(rcs, base, mlog, slog, days)
     check_tracers_and_archives(str(tmp_path), kw)
