@@ -41,13 +41,25 @@ struct Shape {
   }
 };
 
+// Tracers the array holds: the ntracr the call remaps, and the diagnostic
+// tracers 801-803 wherever their indices put them.  The model keeps those
+// after the remapped ones (its mtracr, blkdat.F90:1307-1330), and hybgen
+// writes them without remapping them (hybgen.F90:44-164 beside the
+// do ktr=1,ntracr loops of hybgenaj).
+int tracer_extent(const hyb_params* P) {
+  int e = P->ntracr;
+  const int it[3] = {P->itracr801, P->itracr802, P->itracr803};
+  for (int a = 0; a < 3; a++) e = it[a] > e ? it[a] : e;
+  return e;
+}
+
 Shape shape_of(const hyb_params* P) {
   Shape s;
   s.NI = P->idm + 2 * P->nbdy;
   s.NJ = P->jdm + 2 * P->nbdy;
   s.NIJ = s.NI * s.NJ;
   s.K = P->kdm;
-  s.ntr = P->ntracr;
+  s.ntr = tracer_extent(P);
   return s;
 }
 
@@ -81,7 +93,8 @@ int check_fields(const hyb_params* P, const hyb_fields* F, int raflag) {
                          "vbavg", "ip", "iu", "iv"};
   for (size_t q = 0; q < sizeof need / sizeof need[0]; q++)
     if (!need[q]) return fail(HYB_ENULL, "field %s is NULL", names[q]);
-  if (P->ntracr > 0 && !F->tracer) return fail(HYB_ENULL, "field tracer is NULL (ntracr>0)");
+  if (tracer_extent(P) > 0 && !F->tracer)
+    return fail(HYB_ENULL, "field tracer is NULL (ntracr or itracr > 0)");
   if (raflag && !F->dpo) return fail(HYB_ENULL, "field dpo is NULL (hybgen_raflag set)");
   return 0;
 }
@@ -373,8 +386,8 @@ int hycom_hybgen_check_params(const hyb_params* P) {
                 "isopcm off with nhybrd<kdm carries PCM flags across columns of a row");
   int it[3] = {P->itracr801, P->itracr802, P->itracr803};
   for (int a = 0; a < 3; a++) {
-    if (it[a] < 0 || it[a] > P->ntracr)
-      return fail(HYB_EINVAL, "itracr(%d)=%d outside 0..ntracr", 801 + a, it[a]);
+    if (it[a] < 0 || it[a] > HYB_TRMAX)  // mxtrcr, mod_dimensions.F90:19-21
+      return fail(HYB_EINVAL, "itracr(%d)=%d outside 0..%d", 801 + a, it[a], HYB_TRMAX);
     for (int b = 0; b < a; b++)
       if (it[a] && it[a] == it[b])
         return fail(HYB_EUNSUPPORTED, "itracr(%d) and itracr(%d) share a tracer", 801 + b,
@@ -1010,7 +1023,7 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
     hipError_t e = hipSuccess;
     for (int q = 0; q < 8 && e == hipSuccess; q++)
       e = to_device ? h2d(dev[q] + o, host[q] + o, b3) : d2h(host[q] + o, dev[q] + o, b3);
-    for (int t = 0; t < P->ntracr && e == hipSuccess; t++) {
+    for (int t = 0; t < s.ntr && e == hipSuccess; t++) {
       const size_t ot = n4 * (size_t)t + o;
       e = to_device ? h2d(M.tracer + ot, F->tracer + ot, b3)
                     : d2h(F->tracer + ot, M.tracer + ot, b3);
@@ -1041,7 +1054,7 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
   hyb_fields D;
   D.dp = M.dp; D.dpo = raflag ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
   D.temp = M.temp; D.saln = M.saln; D.th3d = M.th3d; D.u = M.u; D.v = M.v;
-  D.tracer = P->ntracr > 0 ? M.tracer : nullptr;
+  D.tracer = s.ntr > 0 ? M.tracer : nullptr;
   D.p = M.p; D.pu = M.pu; D.pv = M.pv; D.theta = M.theta;
   D.depths = M.depths; D.topiso = M.topiso; D.depthu = M.depthu; D.depthv = M.depthv;
   D.ubavg = M.ubavg; D.vbavg = M.vbavg; D.ip = M.ip; D.iu = M.iu; D.iv = M.iv;

@@ -131,7 +131,7 @@
       fld%th3d   = raddr(th3d)
       fld%u      = raddr(u)
       fld%v      = raddr(v)
-      if (ntracr.gt.0) then
+      if (ntracr+mtracr.gt.0) then  ! mtracr: the diagnostic tracers
         fld%tracer = raddr(tracer)
       else
         fld%tracer = c_null_ptr

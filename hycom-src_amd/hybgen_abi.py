@@ -303,11 +303,18 @@ def make_synth(P, w, seed=100, i0=0, j0=0, itdm=None, jtdm=None, extreme=0):
 # ---------------------------------------------------------------------------
 # field storage
 # ---------------------------------------------------------------------------
+def tracer_extent(P):
+    """Tracers the tracer array holds: the ntracr remapped ones and the
+    diagnostic tracers 801-803 wherever their indices put them (the model
+    keeps them after the ntracr tracers)."""
+    return max(P.ntracr, P.itracr801, P.itracr802, P.itracr803)
+
+
 def field_shapes(P):
     NI = P.idm + 2 * P.nbdy
     NJ = P.jdm + 2 * P.nbdy
     K = P.kdm
-    ntr = max(P.ntracr, 1)
+    ntr = max(tracer_extent(P), 1)
     s4 = (NI, NJ, K, 2)
     return {
         "dp": s4, "dpo": s4, "dpu": s4, "dpv": s4, "temp": s4, "saln": s4,
@@ -344,7 +351,7 @@ class HostDomain:
         F = HybFields()
         for n in FIELD_NAMES:
             setattr(F, n, self.a[n].ctypes.data)
-        if self.P.ntracr == 0:
+        if tracer_extent(self.P) == 0:
             F.tracer = None
         return F
 
@@ -378,7 +385,7 @@ class DeviceDomain:
         F = HybFields()
         for n in FIELD_NAMES:
             setattr(F, n, self.t[n].data_ptr() if n in self.t else None)
-        if self.P.ntracr == 0:
+        if tracer_extent(self.P) == 0:
             F.tracer = None
         return F
 

@@ -92,6 +92,11 @@ typedef struct hyb_params {
   int32_t itracr801;     /* tracer index or 0: dp change diagnostic          */
   int32_t itracr802;     /* tracer index or 0: temp change diagnostic        */
   int32_t itracr803;     /* tracer index or 0: saln change diagnostic        */
+                         /* (1..99; above ntracr a diagnostic tracer is only
+                            written, not remapped, as the model keeps its
+                            mtracr diagnostics after the ntracr tracers,
+                            blkdat.F90:1307-1330; the tracer array then
+                            holds max(ntracr, itracr801..803) tracers)       */
   int32_t precision;     /* HYB_PREC_FP64 or HYB_PREC_MIXED                  */
   int32_t check_finite;  /* logical: scan the inputs the call reads at sea p-,
                             u- and v-points for NaN/Inf first; the first bad
@@ -112,13 +117,14 @@ typedef struct hyb_params {
  * Field pointers, each in the layout described at the top of this file.
  * In hycom_hybgen() they are host pointers; in hycom_hybgen_device() and
  * the batched/serving calls they are device pointers.  Fields the call does
- * not touch may be NULL (dpo unless raflag, tracer when ntracr==0).
+ * not touch may be NULL (dpo unless raflag, tracer when ntracr and the
+ * itracr are all 0).
  */
 typedef struct hyb_fields {
   double *dp, *dpo, *dpu, *dpv;        /* (NI,NJ,kdm,2)                     */
   double *temp, *saln, *th3d;          /* (NI,NJ,kdm,2)                     */
   double *u, *v;                       /* (NI,NJ,kdm,2)                     */
-  double *tracer;                      /* (NI,NJ,kdm,2,ntracr)              */
+  double *tracer;                      /* (NI,NJ,kdm,2,max(ntracr,itracr))  */
   double *p, *pu, *pv;                 /* (NI,NJ,kdm+1)                     */
   const double *theta;                 /* (NI,NJ,kdm)                       */
   const double *depths, *topiso;       /* (NI,NJ); depths in m, topiso Pa   */

@@ -77,7 +77,7 @@
       subroutine ensure_alloc(prm)
       type(hyb_params_t), intent(in) :: prm
       integer :: ntr
-      ntr = max(prm%ntracr, 1)
+      ntr = max(prm%ntracr, prm%itracr801, prm%itracr802, prm%itracr803, 1)
       if (a_idm.eq.prm%idm .and. a_jdm.eq.prm%jdm .and. &
           a_kdm.eq.prm%kdm .and. a_ntr.eq.ntr) return
       if (a_idm.gt.0) then
@@ -158,7 +158,8 @@
       hybmap  = prm%hybmap
       hybflg  = prm%hybflg
       ntracr  = prm%ntracr
-      mtracr  = 0
+      mtracr  = max(prm%itracr801, prm%itracr802, prm%itracr803, &
+                    prm%ntracr) - prm%ntracr
       trcflg(:) = 0
       do k= 1,prm%ntracr
         trcflg(k) = prm%trcflg(k)
@@ -182,7 +183,7 @@
       type(hyb_params_t), intent(in) :: prm
       type(hyb_fields_t), intent(in) :: fld
       integer :: n2, n3, n4, n5, n3p, ntr
-      ntr = max(prm%ntracr, 1)
+      ntr = max(prm%ntracr, prm%itracr801, prm%itracr802, prm%itracr803, 1)
       n2  = (idm+2*nbdy)*(jdm+2*nbdy)
       n3  = n2*kdm
       n3p = n2*(kdm+1)
@@ -200,7 +201,8 @@
       call cin(th3d, fld%th3d, n4)
       call cin(u,    fld%u,    n4)
       call cin(v,    fld%v,    n4)
-      if (prm%ntracr .gt. 0) call cin(tracer, fld%tracer, n5)
+      if (max(prm%ntracr, prm%itracr801, prm%itracr802, prm%itracr803) &
+          .gt. 0) call cin(tracer, fld%tracer, n5)
       call cin(p,  fld%p,  n3p)
       call cin(pu, fld%pu, n3p)
       call cin(pv, fld%pv, n3p)
@@ -217,7 +219,7 @@
       type(hyb_params_t), intent(in) :: prm
       type(hyb_fields_t), intent(in) :: fld
       integer :: n2, n3, n4, n5, n3p, ntr
-      ntr = max(prm%ntracr, 1)
+      ntr = max(prm%ntracr, prm%itracr801, prm%itracr802, prm%itracr803, 1)
       n2  = (idm+2*nbdy)*(jdm+2*nbdy)
       n3  = n2*kdm
       n3p = n2*(kdm+1)
@@ -232,7 +234,8 @@
       call cout(th3d, fld%th3d, n4)
       call cout(u,    fld%u,    n4)
       call cout(v,    fld%v,    n4)
-      if (prm%ntracr .gt. 0) call cout(tracer, fld%tracer, n5)
+      if (max(prm%ntracr, prm%itracr801, prm%itracr802, prm%itracr803) &
+          .gt. 0) call cout(tracer, fld%tracer, n5)
       call cout(p,  fld%p,  n3p)
       call cout(pu, fld%pu, n3p)
       call cout(pv, fld%pv, n3p)

@@ -54,14 +54,17 @@ def _recs(fa, fields, idm, jdm, mask=None):
 
 
 def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0,
-               trcflg=3, days=1.0, periodic=False, nsigma=0, hybthk=0.0, shelf=20.0):
+               trcflg=3, days=1.0, periodic=False, nsigma=0, hybthk=0.0, shelf=20.0,
+               diag=()):
     """Writes the case into directory `path` (created if missing).  The run
     starts at day 1 (step 96 with baclin 900 s) and ends `days` later.
     periodic: a channel periodic in i (sea in the last column makes the
     model take the domain as periodic, bigrid.F90:21-33), land rows at both
     j edges; otherwise a closed basin.  shelf: the shallowest depth, m
     (under the nsigma layers' deep limit dpns the columns are terrain
-    following, geopar.F90:449-630)."""
+    following, geopar.F90:449-630).  diag: diagnostic tracer types (801
+    hybgen's change of dp, 802 of T, 803 of S), kept after the ntracr
+    tracers (mtracr, blkdat.F90:1307-1330)."""
     assert 4 <= kdm <= 64 and idm >= 8 and jdm >= 8
     os.makedirs(path, exist_ok=True)
     P = lambda f: os.path.join(path, f)  # noqa: E731
@@ -182,7 +185,9 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     F("thkfrz", 0.0); I("iceflg", 0); F("tfrz_0", 0.0); F("tfrz_s", -0.054)
     F("frzifq", -1.0); F("ticegr", 0.0); F("hicemn", 0.5); F("hicemx", 10.0)
     # a negative ntracr starts the tracers from relax.trcr (blkdat.F90:1272-1274)
-    I("ishelf", 0); I("ntracr", -ntracr); I("trcflg", trcflg); I("mtracr", 0)
+    I("ishelf", 0); I("ntracr", -ntracr); I("trcflg", trcflg); I("mtracr", len(diag))
+    for d in diag:
+        I("dtrflg", d)
     I("mstrcr", 0); I("tsofrq", 0); F("tofset", 0.0); F("sofset", 0.0); I("mlflag", 0)
     I("pensol", 0); F("dtrate", 0.0); F("thkmin", 0.0); I("dypflg", 0); I("mixfrq", 1)
     F("diapyc", 0.0); F("rigr", 0.25); F("ribc", 0.45); F("rinfty", 0.7)

@@ -484,3 +484,26 @@ def test_zlevel_start_matches_oracle(gpu, kernel_path, hybmap, raflag, hybthk):
         import torch
         torch.cuda.synchronize()
         _assert_same(want, dd.download(), f"zlevel-start hybmap={hybmap} seed={seed}")
+
+
+@pytest.mark.parametrize("ntracr,itr,raflag", [(0, (1, 2, 3), 0), (2, (3, 5, 4), 1)])
+def test_diagnostics_after_remapped_tracers(gpu, kernel_path, ntracr, itr, raflag):
+    """itracr(801..803) above ntracr, where the model keeps its mtracr
+    diagnostic tracers: the device path and copy mode against the oracle."""
+    P, w = _case("diag-after-tracers", idm=48, jdm=36, kdm=22, hybmap=2, ntracr=ntracr,
+                 trcflg=[0, 2][:ntracr])
+    P.itracr801, P.itracr802, P.itracr803 = itr
+    _select(kernel_path, P)
+    d0 = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=50))
+    rng = np.random.default_rng(51)
+    d0.a["tracer"][...] = rng.standard_normal(d0.a["tracer"].shape)
+    if raflag:
+        d0.a["dpo"][...] = np.abs(d0.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d0.a["dp"].shape)))
+    want = _oracle_out(d0, raflag=raflag)
+    dd = abi.DeviceDomain(P).upload(d0)
+    dd.hybgen(m=1, n=2, raflag=raflag)
+    import torch
+    torch.cuda.synchronize()
+    _assert_same(want, dd.download(), f"device, ntracr={ntracr} itracr={itr}")
+    got = d0.copy().hybgen(m=1, n=2, raflag=raflag)
+    _assert_same(want, got, f"copy mode, ntracr={ntracr} itracr={itr}")

@@ -124,3 +124,23 @@ def test_device_code_on_host_zlevel_start(family, hybmap, raflag, hybthk):
         hc.run(family, b, m=m, n=n, raflag=raflag)
         df = ol.diff_domains(a, b)
         assert not df, f"{family} hybmap={hybmap} raflag={raflag} seed={seed}: {df}"
+
+
+@pytest.mark.parametrize("family", ["streaming", "streaming-recompute", "lane-arrays"])
+@pytest.mark.parametrize("ntracr,itr,raflag", [(0, (1, 2, 3), 0), (2, (3, 5, 4), 1)])
+def test_device_code_on_host_diagnostics_after_remapped_tracers(family, ntracr, itr, raflag):
+    """itracr(801..803) above ntracr (the model's mtracr diagnostics):
+    written, not remapped."""
+    P, w = abi.workload_params("cfg1", idm=48, jdm=36, kdm=22, hybmap=2, ntracr=ntracr,
+                               trcflg=[0, 2][:ntracr])
+    P.itracr801, P.itracr802, P.itracr803 = itr
+    d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=50))
+    rng = np.random.default_rng(51)
+    d.a["tracer"][...] = rng.standard_normal(d.a["tracer"].shape)
+    if raflag:
+        d.a["dpo"][...] = np.abs(d.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d.a["dp"].shape)))
+    a, b = d.copy(), d.copy()
+    ol.run_oracle(a, raflag=raflag, threads=4)
+    hc.run(family, b, raflag=raflag)
+    df = ol.diff_domains(a, b)
+    assert not df, f"{family} ntracr={ntracr} itracr={itr}: {df}"

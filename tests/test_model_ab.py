@@ -139,6 +139,8 @@ CASES = [
     ("ppm_sigma_thk", dict(hybmap=2, nsigma=4, hybthk=0.5, shelf=8.0)),
     ("plm_3days", dict(hybmap=1, days=3.0)),
     ("ppm_denfix", dict(hybmap=2, denfix=True)),
+    ("ppm_ra_diag", dict(hybmap=2, hybraf=True, diag=(801, 802, 803))),
+    ("plm_tracer_diag", dict(hybmap=1, ntracr=1, trcflg=3, diag=(803, 801))),
 ]
 
 

@@ -161,3 +161,28 @@ def test_oracle_matches_reference_zlevel_start(kdm, hybmap, raflag, ldenfix, hyb
         ol.run_ref(b, m=mn[0], n=mn[1], raflag=raflag)
         df = ol.diff_domains(a, b)
         assert not df, f"kdm={kdm} hybmap={hybmap} raflag={raflag} m,n={mn}: {df}"
+
+
+@pytest.mark.parametrize("ntracr,itr,raflag", [(0, (1, 2, 3), 0), (2, (3, 5, 4), 1),
+                                               (1, (0, 2, 0), 0), (3, (4, 0, 6), 1)])
+def test_diagnostic_tracers_after_the_remapped_ones(ntracr, itr, raflag):
+    """itracr(801..803) above ntracr, where the model keeps its mtracr
+    diagnostic tracers (blkdat.F90:1307-1330): written before and after
+    hybgenaj (hybgen.F90:44-164) and not remapped, since hybgenaj's tracer
+    loops stop at ntracr."""
+    P, w = abi.workload_params("cfg1", idm=48, jdm=36, kdm=22, hybmap=2, ntracr=ntracr,
+                               trcflg=[0, 2, 0][:ntracr])
+    P.itracr801, P.itracr802, P.itracr803 = itr
+    for mn in ((1, 2), (2, 1)):
+        d = abi.HostDomain(P).synth(abi.make_synth(P, w, seed=40 + ntracr), m=mn[0], n=mn[1])
+        rng = np.random.default_rng(41)
+        d.a["tracer"][...] = rng.standard_normal(d.a["tracer"].shape)
+        for name in ("tracer",):
+            assert d.a[name].shape[-1] == max(ntracr, *itr)
+        if raflag:
+            d.a["dpo"][...] = np.abs(d.a["dp"] * (1.0 + 0.01 * rng.standard_normal(d.a["dp"].shape)))
+        a, b = d.copy(), d.copy()
+        ol.run_oracle(a, m=mn[0], n=mn[1], raflag=raflag, threads=4)
+        ol.run_ref(b, m=mn[0], n=mn[1], raflag=raflag)
+        df = ol.diff_domains(a, b)
+        assert not df, f"ntracr={ntracr} itracr={itr} raflag={raflag} m,n={mn}: {df}"
