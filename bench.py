@@ -546,6 +546,64 @@ def serve_leg(args):
                          f"jdm 100-400, 4 tracers, PPM; nsteps U{{1..10}}; FP64 or mixed 50/50")}
 
 
+def model_timing(args):
+    """The stand-alone model, stock and with the drop-in, on one case: a
+    300x240x41 closed basin at 0.1 degree (tests/model_case.py), 450 s
+    steps for an eighth of a day (inicon's two hybgen calls and 24 steps).  The
+    builds are serial (config/generic-gnu-relo_one); the drop-in runs its
+    hybgen on the GPU in copy mode.  Reports the model's own timers
+    (xctmr: hybgen is timer 49 around the call, mod_hycom.F90:2648-2653)
+    and whether the two runs' archives are the same bytes."""
+    import re
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import model_case
+    mdl = os.path.join(ROOT, "oracle", "_ref", "model")
+    case = dict(idm=300, jdm=240, kdm=41, hybmap=2, days=0.125, res=0.1, baclin=450.0,
+                batrop=15.0)
+    out = {"case": "300x240x41 closed basin, 0.1 degree, z-level start, PPM, "
+                   "inicon's 2 hybgen calls + 24 steps of 450 s; serial model builds",
+           "runs": {}}
+    dirs = {}
+    for name, exe in (("stock", "hycom"), ("dropin", "hycom_hip")):
+        path = os.path.join(mdl, exe)
+        if not os.path.exists(path):
+            raise SystemExit(f"bench.py --model-timing: {path} is not built "
+                             "(make -C oracle/ref model)")
+        d = tempfile.mkdtemp(prefix=f"model_{name}_")
+        model_case.write_case(d, **case)
+        print(f"[bench] model timing: {name} ({exe})", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        with open(os.path.join(d, "run.log"), "w") as log:
+            rc = subprocess.run([path], cwd=d, stdout=log, stderr=subprocess.STDOUT,
+                                timeout=900).returncode
+        wall = time.perf_counter() - t0
+        text = open(os.path.join(d, "run.log")).read()
+        if rc != 0 or "(normal)" not in text:
+            raise SystemExit(f"bench.py --model-timing: {name} run failed (rc {rc})\n"
+                             + text[-2000:])
+        block = text[text.rindex("timer statistics"):]
+        timers = {m.group(1): {"calls": int(m.group(2)), "s": float(m.group(3))}
+                  for m in re.finditer(r"^\s+(\S+)\s+calls =\s+(\d+)\s+time =\s+([\d.]+)",
+                                       block, re.M)}
+        out["runs"][name] = {"wall_s": wall, "timers": timers}
+        dirs[name] = d
+        print(f"[bench] model timing: {name} hybgen {timers.get('hybgen')}, "
+              f"total {timers.get('total')}", file=sys.stderr, flush=True)
+    arch = sorted(f for f in os.listdir(dirs["stock"]) if f.startswith("archv.") and f.endswith(".a"))
+    out["archives_identical"] = bool(arch) and all(
+        open(os.path.join(dirs["stock"], f), "rb").read()
+        == open(os.path.join(dirs["dropin"], f), "rb").read() for f in arch)
+    hs, hd = (out["runs"][n]["timers"]["hybgen"] for n in ("stock", "dropin"))
+    out["hybgen_ms_per_call"] = {"stock": 1e3 * hs["s"] / hs["calls"],
+                                 "dropin": 1e3 * hd["s"] / hd["calls"]}
+    out["hybgen_speedup"] = hs["s"] / hd["s"]
+    out["model_speedup"] = (out["runs"]["stock"]["timers"]["total"]["s"]
+                            / out["runs"]["dropin"]["timers"]["total"]["s"])
+    print(json.dumps({"model_timing": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -573,6 +631,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the HBM-counter calibration kernels (1 GiB fill, sum, clone)")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-mode leg")
+    ap.add_argument("--model-timing", action="store_true",
+                    help="only time the stand-alone model, stock and with the drop-in "
+                         "(oracle/_ref/model, tests/model_case.py)")
     ap.add_argument("--no-denfix", action="store_true", help="skip the HYBGEN_DENFIX leg")
     ap.add_argument("--no-raflag", action="store_true", help="skip the Robert-Asselin leg")
     ap.add_argument("--copy-idm", type=int, default=1500)
@@ -602,6 +663,10 @@ def main():
               "(the hybgen path is single-GPU here; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.model_timing:
+        model_timing(args)
+        return
 
     import torch
     if args.calibrate:

@@ -55,7 +55,8 @@ def _recs(fa, fields, idm, jdm, mask=None):
 
 def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0,
                trcflg=3, days=1.0, periodic=False, nsigma=0, hybthk=0.0, shelf=20.0,
-               diag=()):
+               diag=(), hybiso=0.01, hybrlx=8.0, dx00=None, res=0.5, baclin=900.0,
+               batrop=30.0):
     """Writes the case into directory `path` (created if missing).  The run
     starts at day 1 (step 96 with baclin 900 s) and ends `days` later.
     periodic: a channel periodic in i (sea in the last column makes the
@@ -64,21 +65,25 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     (under the nsigma layers' deep limit dpns the columns are terrain
     following, geopar.F90:449-630).  diag: diagnostic tracer types (801
     hybgen's change of dp, 802 of T, 803 of S), kept after the ntracr
-    tracers (mtracr, blkdat.F90:1307-1330)."""
+    tracers (mtracr, blkdat.F90:1307-1330).  hybiso 0 turns the PCM
+    remap of near-isopycnal layers off (isopcm); dx00=(dx00, dx00x, dx00f)
+    sets maximum layer thicknesses (the dx00 form of blkdat.F90:234-250).
+    res: grid spacing in degrees (from 15N, 60W)."""
     assert 4 <= kdm <= 64 and idm >= 8 and jdm >= 8
     os.makedirs(path, exist_ok=True)
     P = lambda f: os.path.join(path, f)  # noqa: E731
 
-    # grid: 0.5 degree, 15N.., 60W..
+    # grid: res degrees, 15N.., 60W..
+    assert 15.0 + res * jdm < 80.0
     ii = np.arange(idm)[None, :] * np.ones((jdm, 1))
     jj = np.arange(jdm)[:, None] * np.ones((1, idm))
-    plon, plat = -60.0 + 0.5 * ii, 15.0 + 0.5 * jj
-    rad, R = math.pi / 180.0, 6371001.0
-    scx = lambda lat: R * rad * 0.5 * np.cos(lat * rad)  # noqa: E731
-    scy = lambda lat: R * rad * 0.5 * np.ones_like(lat)  # noqa: E731
-    qlat, ulat, vlat = plat - 0.25, plat, plat - 0.25
-    grid = [("plon", plon), ("plat", plat), ("qlon", plon - 0.25), ("qlat", qlat),
-            ("ulon", plon - 0.25), ("ulat", ulat), ("vlon", plon), ("vlat", vlat),
+    plon, plat = -60.0 + res * ii, 15.0 + res * jj
+    rad, R, h = math.pi / 180.0, 6371001.0, 0.5 * res
+    scx = lambda lat: R * rad * res * np.cos(lat * rad)  # noqa: E731
+    scy = lambda lat: R * rad * res * np.ones_like(lat)  # noqa: E731
+    qlat, ulat, vlat = plat - h, plat, plat - h
+    grid = [("plon", plon), ("plat", plat), ("qlon", plon - h), ("qlat", qlat),
+            ("ulon", plon - h), ("ulat", ulat), ("vlon", plon), ("vlat", vlat),
             ("pang", np.zeros_like(plon)),
             ("pscx", scx(plat)), ("pscy", scy(plat)), ("qscx", scx(qlat)), ("qscy", scy(qlat)),
             ("uscx", scx(ulat)), ("uscy", scy(ulat)), ("vscx", scx(vlat)), ("vscy", scy(vlat)),
@@ -161,6 +166,8 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     # minimum thicknesses dp0k = dp00*dp00f**(k-1) up to dp00x; with many
     # layers a smaller start and cap keep their sum a few hundred metres
     dp00, dp00x = (3.0, 450.0) if kdm <= len(ZI) else (1.0, 12.0)
+    if dx00:
+        F("dx00", dx00[0]); F("dx00x", dx00[1]); F("dx00f", dx00[2])
     F("dp00", dp00); F("dp00x", dp00x); F("dp00f", 1.18)
     F("ds00", 1.0); F("ds00x", 15.0); F("ds00f", 1.18); F("dp00i", 1.0); F("isotop", 6.0)
     F("oneta0", 0.1); F("saln0", 35.0); I("locsig", 0); I("kapref", 0); I("thflag", 2)
@@ -170,11 +177,11 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     I("iniflg", 2); I("jerlv0", 1); I("yrflag", 3); I("sshflg", 0)
     F("dsurfq", 1.0); F("diagfq", days); F("proffq", 0.0); F("tilefq", 0.0)
     F("meanfq", 0.0); F("rstrfq", 365.0); F("bnstfq", 0.0); F("nestfq", 0.0)
-    F("cplifq", -1.0); F("baclin", 900.0); F("batrop", 30.0)
+    F("cplifq", -1.0); F("baclin", baclin); F("batrop", batrop)
     I("incflg", 0); I("incstp", 0); I("incupf", 0)
     I("stfflg", 0); F("stfrdt", 0.0); F("stfrds", 0.0); F("stfrdv", 0.0)
     F("ra2fac", 0.125); F("wbaro", 0.125); I("btrlfr", 1); I("btrmas", 0)
-    I("hybraf", hybraf); F("hybrlx", 8.0); F("hybiso", 0.01); F("hybthn", 1.0)
+    I("hybraf", hybraf); F("hybrlx", hybrlx); F("hybiso", hybiso); F("hybthn", 1.0)
     F("hybthk", hybthk); I("hybmap", hybmap); I("hybflg", 0); I("advflg", 0)
     I("advtyp", 2); I("momtyp", 2); F("slip", 0.0); F("visco2", 0.05); F("visco4", 0.2)
     F("facdf4", 0.0); F("veldf2", 0.02); F("veldf4", 0.01); F("thkdf2", 0.0)

@@ -141,6 +141,8 @@ CASES = [
     ("ppm_denfix", dict(hybmap=2, denfix=True)),
     ("ppm_ra_diag", dict(hybmap=2, hybraf=True, diag=(801, 802, 803))),
     ("plm_tracer_diag", dict(hybmap=1, ntracr=1, trcflg=3, diag=(803, 801))),
+    ("ppm_dx0k_noiso_norelax", dict(hybmap=2, dx00=(30.0, 300.0, 1.1), hybiso=0.0,
+                                    hybrlx=1.0)),
 ]
 
 
