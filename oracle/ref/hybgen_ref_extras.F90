@@ -147,6 +147,9 @@
       prm%jdm = jdm_
       prm%kdm = 1
       prm%ntracr = 0
+      prm%itracr801 = 0
+      prm%itracr802 = 0
+      prm%itracr803 = 0
       call ensure_alloc(prm)
       lp = 6
       mnproc = 1
