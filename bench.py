@@ -344,7 +344,7 @@ def copy_leg(args):
     model's call takes) on a window of the GLBb field: ms per call with the
     PCIe transfers, the bytes each way, and the same window's device-only
     call for comparison.  The first call allocates the library's device
-    mirror and is not timed."""
+    mirror and pinned staging buffers and is not timed."""
     import torch
     lib = abi.load()
     P, w = abi.workload_params(args.workload, idm=args.copy_idm, jdm=args.copy_jdm,
@@ -369,7 +369,7 @@ def copy_leg(args):
         td.append(time.perf_counter() - ts)
     del dd
     torch.cuda.empty_cache()
-    h.hybgen(m=1, n=2)  # allocates the device mirror
+    h.hybgen(m=1, n=2)  # allocates the device mirror and staging buffers
     tc = []
     for q in range(3):
         ts = time.perf_counter()
@@ -386,9 +386,9 @@ def copy_leg(args):
             "column_layers_per_s": sea * P.kdm / (ms * 1e-3),
             "bytes_up": up.value, "bytes_down": down.value,
             "pcie_GBs": (up.value + down.value) / (ms * 1e-3) / 1e9,
-            "about": "median of 3 calls after one that allocates the device mirror; levels "
-                     "n of the fields the call touches go both ways, p/pu/pv both ways, the "
-                     "read-only fields up, on every call"}
+            "about": "median of 3 calls after one that allocates the device mirror and the "
+                     "pinned staging buffers; levels n of the fields the call touches go both "
+                     "ways, p/pu/pv both ways, the read-only fields up, on every call"}
 
 
 def batch_leg(args):

@@ -9,6 +9,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include <mutex>
 #include <vector>
 
@@ -207,6 +209,128 @@ struct Mirror {
 
 std::mutex g_mirror_mu;
 Mirror g_mirror;
+
+// Copy mode's transfers of pageable host arrays go through two pinned
+// staging buffers: the CPU copies a piece into one buffer (threads split
+// large pieces) while the DMA engine moves the other, so uploads and
+// downloads run near the PCIe rate.  The runtime's own path for pageable
+// memory ran at about 27 GB/s up and 17 GB/s down on MI355X, the staged one
+// at about 54 GB/s both ways.  Pieces are queued in order on one stream; a
+// buffer is written again only after the event recorded behind its last
+// DMA, and the downloads queued into it are copied out to the host then.
+struct Stager {
+  static constexpr size_t kCap = size_t(32) << 20;
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool armed[2] = {false, false};
+  struct Out {
+    void* host;
+    size_t off, n;
+  };
+  std::vector<Out> pending[2];
+  int q = 0;
+  size_t fill = 0;
+  hipStream_t st = nullptr;
+
+  static void pcopy(void* d, const void* s_, size_t n) {
+    if (n < (size_t(4) << 20)) {
+      memcpy(d, s_, n);
+      return;
+    }
+    int nt = omp_get_max_threads();
+    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int t = 0; t < nt; t++) {
+      const size_t a = n * (size_t)t / (size_t)nt, b = n * (size_t)(t + 1) / (size_t)nt;
+      memcpy((char*)d + a, (const char*)s_ + a, b - a);
+    }
+  }
+  hipError_t begin(hipStream_t s_) {
+    st = s_;
+    q = 0;
+    fill = 0;
+    for (int b = 0; b < 2; b++) {
+      if (!buf[b]) {
+        hipError_t e = hipHostMalloc((void**)&buf[b], kCap, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+      }
+      if (!ev[b]) {
+        hipError_t e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+      }
+      armed[b] = false;
+      pending[b].clear();
+    }
+    return hipSuccess;
+  }
+  // buffer b is free again: its DMA done, its downloads copied out
+  hipError_t reuse(int b) {
+    if (armed[b]) {
+      hipError_t e = hipEventSynchronize(ev[b]);
+      if (e != hipSuccess) return e;
+      armed[b] = false;
+    }
+    for (const Out& o : pending[b]) pcopy(o.host, buf[b] + o.off, o.n);
+    pending[b].clear();
+    return hipSuccess;
+  }
+  hipError_t next() {
+    hipError_t e = hipEventRecord(ev[q], st);
+    if (e != hipSuccess) return e;
+    armed[q] = true;
+    q ^= 1;
+    fill = 0;
+    return reuse(q);
+  }
+  hipError_t up(void* dev, const void* host, size_t n) {
+    while (n > 0) {
+      if (fill == kCap) {
+        hipError_t e = next();
+        if (e != hipSuccess) return e;
+      }
+      const size_t m = n < kCap - fill ? n : kCap - fill;
+      pcopy(buf[q] + fill, host, m);
+      hipError_t e = hipMemcpyAsync(dev, buf[q] + fill, m, hipMemcpyHostToDevice, st);
+      if (e != hipSuccess) return e;
+      fill += m;
+      dev = (char*)dev + m;
+      host = (const char*)host + m;
+      n -= m;
+    }
+    return hipSuccess;
+  }
+  hipError_t down(void* host, const void* dev, size_t n) {
+    while (n > 0) {
+      if (fill == kCap) {
+        hipError_t e = next();
+        if (e != hipSuccess) return e;
+      }
+      const size_t m = n < kCap - fill ? n : kCap - fill;
+      hipError_t e = hipMemcpyAsync(buf[q] + fill, dev, m, hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return e;
+      pending[q].push_back({host, fill, m});
+      fill += m;
+      dev = (const char*)dev + m;
+      host = (char*)host + m;
+      n -= m;
+    }
+    return hipSuccess;
+  }
+  // everything queued has landed and every download is in host memory
+  hipError_t finish() {
+    hipError_t e = hipEventRecord(ev[q], st);
+    if (e != hipSuccess) return e;
+    armed[q] = true;
+    for (int b = 0; b < 2; b++) {
+      const int x = (q + 1 + b) & 1;  // the older buffer first
+      if ((e = reuse(x)) != hipSuccess) return e;
+    }
+    fill = 0;
+    return hipSuccess;
+  }
+};
+
+Stager g_stager;  // under g_mirror_mu; its pinned buffers live until the process ends
 
 // measurement: per-phase HIP events of the calls since the last read
 std::mutex g_timing_mu;
@@ -1005,14 +1129,16 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
     M.shape = s;
   }
   hipStream_t st = M.stream;
+  Stager& X = g_stager;
+  HIPCHK(X.begin(st));
   int64_t up = 0, down = 0;
   auto h2d = [&](void* d, const void* h, size_t bytes) {
     up += (int64_t)bytes;
-    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st);
+    return X.up(d, h, bytes);
   };
   auto d2h = [&](void* h, const void* d, size_t bytes) {
     down += (int64_t)bytes;
-    return hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st);
+    return X.down(h, d, bytes);
   };
   // the fields of time level l (1-based) that the call reads and writes
   const size_t b3 = n3 * sizeof(double);
@@ -1068,6 +1194,7 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
   HIPCHK(d2h(F->p, M.p, n3p * sizeof(double)));
   HIPCHK(d2h(F->pu, M.pu, n3p * sizeof(double)));
   HIPCHK(d2h(F->pv, M.pv, n3p * sizeof(double)));
+  HIPCHK(X.finish());
   HIPCHK(hipStreamSynchronize(st));
   M.up = up;
   M.down = down;

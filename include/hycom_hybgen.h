@@ -180,7 +180,9 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F,
  * pv both ways, and theta, depths, topiso, depthu, depthv and the masks
  * up.  Every call moves all of it: the model changes some of these arrays
  * in place between calls (depthu/depthv after inicon's calls, the halo of
- * p).  hycom_hybgen_copy_invalidate() has nothing to drop and returns 0
+ * p).  The copies go through two pinned staging buffers of the library's
+ * (2 x 32 MiB, allocated on the first copy-mode call and kept until the
+ * process ends).  hycom_hybgen_copy_invalidate() has nothing to drop and returns 0
  * (kept for callers of the earlier cached form).  hycom_hybgen_copy_stats()
  * gives the bytes the last copy-mode call moved each way.  No reference
  * counterpart (the reference has no device copies). */
