@@ -549,11 +549,13 @@ def serve_leg(args):
 def model_timing(args):
     """The stand-alone model, stock and with the drop-in, on one case: a
     300x240x41 closed basin at 0.1 degree (tests/model_case.py), 450 s
-    steps for an eighth of a day (inicon's two hybgen calls and 24 steps).  The
-    builds are serial (config/generic-gnu-relo_one); the drop-in runs its
-    hybgen on the GPU in copy mode.  Reports the model's own timers
-    (xctmr: hybgen is timer 49 around the call, mod_hycom.F90:2648-2653)
-    and whether the two runs' archives are the same bytes."""
+    steps for an eighth of a day (inicon's two hybgen calls and 24 steps).
+    Three builds: stock serial (config/generic-gnu-relo_one), stock with
+    OpenMP over j on 8 threads (relo_omp; mod_dimensions.F90 takes up to 8
+    with RELO), and serial with the drop-in, whose hybgen runs on the GPU in
+    copy mode.  Reports the model's own timers (xctmr: hybgen is timer 49
+    around the call, mod_hycom.F90:2648-2653) and whether the runs'
+    archives are the same bytes."""
     import re
     import subprocess
     import tempfile
@@ -563,10 +565,11 @@ def model_timing(args):
     case = dict(idm=300, jdm=240, kdm=41, hybmap=2, days=0.125, res=0.1, baclin=450.0,
                 batrop=15.0)
     out = {"case": "300x240x41 closed basin, 0.1 degree, z-level start, PPM, "
-                   "inicon's 2 hybgen calls + 24 steps of 450 s; serial model builds",
+                   "inicon's 2 hybgen calls + 24 steps of 450 s",
            "runs": {}}
     dirs = {}
-    for name, exe in (("stock", "hycom"), ("dropin", "hycom_hip")):
+    for name, exe, nthr in (("stock", "hycom", 1), ("stock_omp8", "omp/hycom", 8),
+                            ("dropin", "hycom_hip", 1)):
         path = os.path.join(mdl, exe)
         if not os.path.exists(path):
             raise SystemExit(f"bench.py --model-timing: {path} is not built "
@@ -575,8 +578,9 @@ def model_timing(args):
         model_case.write_case(d, **case)
         print(f"[bench] model timing: {name} ({exe})", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthr))
         with open(os.path.join(d, "run.log"), "w") as log:
-            rc = subprocess.run([path], cwd=d, stdout=log, stderr=subprocess.STDOUT,
+            rc = subprocess.run([path], cwd=d, stdout=log, stderr=subprocess.STDOUT, env=env,
                                 timeout=900).returncode
         wall = time.perf_counter() - t0
         text = open(os.path.join(d, "run.log")).read()
@@ -587,20 +591,22 @@ def model_timing(args):
         timers = {m.group(1): {"calls": int(m.group(2)), "s": float(m.group(3))}
                   for m in re.finditer(r"^\s+(\S+)\s+calls =\s+(\d+)\s+time =\s+([\d.]+)",
                                        block, re.M)}
-        out["runs"][name] = {"wall_s": wall, "timers": timers}
+        out["runs"][name] = {"threads": nthr, "wall_s": wall, "timers": timers}
         dirs[name] = d
         print(f"[bench] model timing: {name} hybgen {timers.get('hybgen')}, "
               f"total {timers.get('total')}", file=sys.stderr, flush=True)
     arch = sorted(f for f in os.listdir(dirs["stock"]) if f.startswith("archv.") and f.endswith(".a"))
     out["archives_identical"] = bool(arch) and all(
         open(os.path.join(dirs["stock"], f), "rb").read()
-        == open(os.path.join(dirs["dropin"], f), "rb").read() for f in arch)
-    hs, hd = (out["runs"][n]["timers"]["hybgen"] for n in ("stock", "dropin"))
-    out["hybgen_ms_per_call"] = {"stock": 1e3 * hs["s"] / hs["calls"],
-                                 "dropin": 1e3 * hd["s"] / hd["calls"]}
-    out["hybgen_speedup"] = hs["s"] / hd["s"]
-    out["model_speedup"] = (out["runs"]["stock"]["timers"]["total"]["s"]
-                            / out["runs"]["dropin"]["timers"]["total"]["s"])
+        == open(os.path.join(dirs[n], f), "rb").read()
+        for f in arch for n in ("stock_omp8", "dropin"))
+    tm = {n: out["runs"][n]["timers"] for n in dirs}
+    out["hybgen_ms_per_call"] = {n: 1e3 * t["hybgen"]["s"] / t["hybgen"]["calls"]
+                                 for n, t in tm.items()}
+    out["hybgen_speedup"] = {n: tm[n]["hybgen"]["s"] / tm["dropin"]["hybgen"]["s"]
+                             for n in ("stock", "stock_omp8")}
+    out["model_speedup"] = {n: tm[n]["total"]["s"] / tm["dropin"]["total"]["s"]
+                            for n in ("stock", "stock_omp8")}
     print(json.dumps({"model_timing": out}), flush=True)
 
 
