@@ -550,10 +550,10 @@ def model_timing(args):
     """The stand-alone model, stock and with the drop-in, on one case: a
     300x240x41 closed basin at 0.1 degree (tests/model_case.py), 450 s
     steps for an eighth of a day (inicon's two hybgen calls and 24 steps).
-    Three builds: stock serial (config/generic-gnu-relo_one), stock with
-    OpenMP over j on 8 threads (relo_omp; mod_dimensions.F90 takes up to 8
-    with RELO), and serial with the drop-in, whose hybgen runs on the GPU in
-    copy mode.  Reports the model's own timers (xctmr: hybgen is timer 49
+    Four builds: stock and with the drop-in (whose hybgen runs on the GPU in
+    copy mode), each serial (config/generic-gnu-relo_one) and with OpenMP
+    over j on 8 threads (relo_omp; mod_dimensions.F90 takes up to 8 with
+    RELO).  Reports the model's own timers (xctmr: hybgen is timer 49
     around the call, mod_hycom.F90:2648-2653) and whether the runs'
     archives are the same bytes."""
     import re
@@ -568,8 +568,10 @@ def model_timing(args):
                    "inicon's 2 hybgen calls + 24 steps of 450 s",
            "runs": {}}
     dirs = {}
-    for name, exe, nthr in (("stock", "hycom", 1), ("stock_omp8", "omp/hycom", 8),
-                            ("dropin", "hycom_hip", 1)):
+    # the serial builds keep the environment's thread count: the drop-in's
+    # library splits its staging copies over those threads
+    for name, exe, nthr in (("stock", "hycom", None), ("dropin", "hycom_hip", None),
+                            ("stock_omp8", "omp/hycom", 8), ("dropin_omp8", "omp/hycom_hip", 8)):
         path = os.path.join(mdl, exe)
         if not os.path.exists(path):
             raise SystemExit(f"bench.py --model-timing: {path} is not built "
@@ -578,7 +580,7 @@ def model_timing(args):
         model_case.write_case(d, **case)
         print(f"[bench] model timing: {name} ({exe})", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        env = dict(os.environ, OMP_NUM_THREADS=str(nthr))
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthr)) if nthr else None
         with open(os.path.join(d, "run.log"), "w") as log:
             rc = subprocess.run([path], cwd=d, stdout=log, stderr=subprocess.STDOUT, env=env,
                                 timeout=900).returncode
@@ -599,14 +601,16 @@ def model_timing(args):
     out["archives_identical"] = bool(arch) and all(
         open(os.path.join(dirs["stock"], f), "rb").read()
         == open(os.path.join(dirs[n], f), "rb").read()
-        for f in arch for n in ("stock_omp8", "dropin"))
+        for f in arch for n in dirs if n != "stock")
     tm = {n: out["runs"][n]["timers"] for n in dirs}
     out["hybgen_ms_per_call"] = {n: 1e3 * t["hybgen"]["s"] / t["hybgen"]["calls"]
                                  for n, t in tm.items()}
-    out["hybgen_speedup"] = {n: tm[n]["hybgen"]["s"] / tm["dropin"]["hybgen"]["s"]
-                             for n in ("stock", "stock_omp8")}
-    out["model_speedup"] = {n: tm[n]["total"]["s"] / tm["dropin"]["total"]["s"]
-                            for n in ("stock", "stock_omp8")}
+    out["hybgen_speedup"] = {"serial": tm["stock"]["hybgen"]["s"] / tm["dropin"]["hybgen"]["s"],
+                             "omp8": tm["stock_omp8"]["hybgen"]["s"]
+                             / tm["dropin_omp8"]["hybgen"]["s"]}
+    out["model_speedup"] = {"serial": tm["stock"]["total"]["s"] / tm["dropin"]["total"]["s"],
+                            "omp8": tm["stock_omp8"]["total"]["s"]
+                            / tm["dropin_omp8"]["total"]["s"]}
     print(json.dumps({"model_timing": out}), flush=True)
 
 
