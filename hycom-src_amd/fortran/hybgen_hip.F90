@@ -96,7 +96,9 @@
       prm%itracr801 = itracr(801)
       prm%itracr802 = itracr(802)
       prm%itracr803 = itracr(803)
-      prm%precision    = 0
+! HYCOM_HYBGEN_PRECISION=mixed in the environment selects the mixed form
+! (HYB_PREC_MIXED); otherwise FP64, bitwise the reference's
+      prm%precision    = hybgen_precision()
       prm%check_finite = 0
       prm%trcflg(:) = 0
       do k= 1,min(ntracr,99)
@@ -150,6 +152,21 @@
       fld%iu     = iaddr(iu)
       fld%iv     = iaddr(iv)
       end subroutine hyb_pack
+
+! 1 when HYCOM_HYBGEN_PRECISION is 'mixed', else 0; read on the first call
+      integer function hybgen_precision()
+      integer, save :: prec = -1
+      character(len=16) :: val
+      integer :: lval, ios
+      if (prec.lt.0) then
+        prec = 0
+        call get_environment_variable('HYCOM_HYBGEN_PRECISION', val, lval, ios)
+        if (ios.eq.0 .and. lval.gt.0) then
+          if (val(1:lval).eq.'mixed') prec = 1
+        endif
+      endif
+      hybgen_precision = prec
+      end function hybgen_precision
 
       end module hybgen_hip_abi
 

@@ -198,3 +198,57 @@ def test_stock_against_shim_over_oracle(tmp_path, name, kw):
     rcs, base, mlog, slog = run_pair(str(tmp_path), stock, dropin, dict(case_kw(kw), days=days))
     assert_identical(rcs, base, mlog, slog, days)
     check_tracers_and_archives(str(tmp_path), kw)
+
+
+def archive_records(path, idm, jdm):
+    """An archive's records (float32, voids as NaN) with the .b line that
+    names each (mod_archiv.F90 writes one .b line per .a record after the
+    header lines)."""
+    import numpy as np
+    n = idm * jdm
+    nw = ((n + 4095) // 4096) * 4096
+    a = np.fromfile(path, dtype=">f4").reshape(-1, nw)[:, :n].astype(np.float64)
+    a[a > 1e29] = np.nan
+    lines = [ln for ln in open(path[:-2] + ".b") if "=" in ln and ":" in ln]
+    return a, lines[-a.shape[0]:]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="oracle/_ref/model/hycom_hip not built (make -C oracle/ref model)")
+def test_mixed_dropin_stays_near_the_stock_model(tmp_path):
+    """HYCOM_HYBGEN_PRECISION=mixed: the drop-in runs the mixed form in the
+    model.  Its interfaces are bitwise the FP64 ones per call and its T, S,
+    tracers and velocities within ~1e-6 of them, so after a day the model
+    state stays near the stock one: every archived field within a small
+    fraction of its range."""
+    import numpy as np
+    print("\n[case] stock model against the drop-in in mixed precision, PPM, 1 day", flush=True)
+    kw = dict(hybmap=2, ntracr=1, trcflg=3, days=1.0)
+    runs = {}
+    for name, exe, env in (("stock", STOCK, {}),
+                           ("mixed", DROPIN, {"HYCOM_HYBGEN_PRECISION": "mixed"})):
+        d = os.path.join(str(tmp_path), name)
+        model_case.write_case(d, **kw)
+        with open(os.path.join(d, "run.log"), "w") as log:
+            rc = subprocess.run([exe], cwd=d, stdout=log, stderr=subprocess.STDOUT,
+                                env=dict(os.environ, **env), timeout=90).returncode
+        text = open(os.path.join(d, "run.log")).read()
+        assert rc == 0 and "(normal)" in text, text[-1500:]
+        runs[name] = d
+    last = sorted(f for f in os.listdir(runs["stock"]) if f.startswith("archv.") and f.endswith(".a"))[-1]
+    a, names = archive_records(os.path.join(runs["stock"], last), 40, 30)
+    b, _ = archive_records(os.path.join(runs["mixed"], last), 40, 30)
+    assert a.shape == b.shape and np.array_equal(np.isnan(a), np.isnan(b))
+    worst = []
+    for r in range(a.shape[0]):
+        x, y = a[r], b[r]
+        ok = ~np.isnan(x)
+        if not ok.any():
+            continue
+        scale = max(np.nanmax(np.abs(x)), 1e-30)
+        worst.append((float(np.max(np.abs(x[ok] - y[ok]))) / scale, names[r].split(":")[0].strip()))
+    worst.sort(reverse=True)
+    print(f"[mixed] largest archived differences over each field's magnitude: {worst[:6]}", flush=True)
+    assert worst[0][0] < 1e-3, worst[:6]
+    assert any(w > 0.0 for w, _ in worst)  # the mixed form did run
