@@ -209,8 +209,11 @@ def archive_records(path, idm, jdm):
     nw = ((n + 4095) // 4096) * 4096
     a = np.fromfile(path, dtype=">f4").reshape(-1, nw)[:, :n].astype(np.float64)
     a[a > 1e29] = np.nan
-    lines = [ln for ln in open(path[:-2] + ".b") if "=" in ln and ":" in ln]
-    return a, lines[-a.shape[0]:]
+    lines = open(path[:-2] + ".b").read().splitlines()
+    first = next(q for q, ln in enumerate(lines) if ln.startswith("field")) + 1
+    names = [ln.split()[0] + " k=" + ln.split("=")[1].split()[2] for ln in lines[first:] if ln.strip()]
+    assert len(names) == a.shape[0], (len(names), a.shape)
+    return a, names
 
 
 @pytest.mark.gpu
@@ -247,7 +250,7 @@ def test_mixed_dropin_stays_near_the_stock_model(tmp_path):
         if not ok.any():
             continue
         scale = max(np.nanmax(np.abs(x)), 1e-30)
-        worst.append((float(np.max(np.abs(x[ok] - y[ok]))) / scale, names[r].split(":")[0].strip()))
+        worst.append((float(np.max(np.abs(x[ok] - y[ok]))) / scale, names[r]))
     worst.sort(reverse=True)
     print(f"[mixed] largest archived differences over each field's magnitude: {worst[:6]}", flush=True)
     assert worst[0][0] < 1e-3, worst[:6]
