@@ -56,7 +56,7 @@ def _recs(fa, fields, idm, jdm, mask=None):
 def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0,
                trcflg=3, days=1.0, periodic=False, nsigma=0, hybthk=0.0, shelf=20.0,
                diag=(), hybiso=0.01, hybrlx=8.0, dx00=None, res=0.5, baclin=900.0,
-               batrop=30.0):
+               batrop=30.0, tnudge=0):
     """Writes the case into directory `path` (created if missing).  The run
     starts at day 1 (step 96 with baclin 900 s) and ends `days` later.
     periodic: a channel periodic in i (sea in the last column makes the
@@ -68,7 +68,10 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
     tracers (mtracr, blkdat.F90:1307-1330).  hybiso 0 turns the PCM
     remap of near-isopycnal layers off (isopcm); dx00=(dx00, dx00x, dx00f)
     sets maximum layer thicknesses (the dx00 form of blkdat.F90:234-250).
-    res: grid spacing in degrees (from 15N, 60W)."""
+    res: grid spacing in degrees (from 15N, 60W).  tnudge: the T climatology
+    moved by that many float32 ulps everywhere (a perturbation at the
+    precision of the input files, to measure how far the model's own
+    sensitivity carries a tiny difference)."""
     assert 4 <= kdm <= 64 and idm >= 8 and jdm >= 8
     os.makedirs(path, exist_ok=True)
     P = lambda f: os.path.join(path, f)  # noqa: E731
@@ -148,6 +151,10 @@ def write_case(path, *, idm=40, jdm=30, kdm=16, hybmap=2, hybraf=False, ntracr=0
                         (10.0 + 3.0 * q) * np.exp(-zmid[k] / (150.0 + 200.0 * q))
                         * (1.0 + 0.3 * np.cos(0.2 * ii + q)) + 0.01 * (100.0 * yj + q)
                         for q in range(ntracr) for k in range(kdm)]
+    if tnudge:
+        clim["temp"] = [np.nextafter(np.asarray(t, np.float32),
+                                     np.float32(np.inf) if tnudge > 0 else np.float32(-np.inf))
+                        .astype(np.float64) for t in clim["temp"]]
     for name, fl in clim.items():
         with open(P(f"relax.{name}.a"), "wb") as fa, open(P(f"relax.{name}.b"), "w") as fb:
             fb.write(f"z-level {name} climatology\n\n\n\n\n")

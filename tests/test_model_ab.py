@@ -216,42 +216,57 @@ def archive_records(path, idm, jdm):
     return a, names
 
 
+def record_spread(d0, d1, idm, jdm):
+    """Per archived record of the last archive: the largest difference
+    between two runs over the record's largest magnitude, sorted."""
+    import numpy as np
+    last = sorted(f for f in os.listdir(d0) if f.startswith("archv.") and f.endswith(".a"))[-1]
+    a, names = archive_records(os.path.join(d0, last), idm, jdm)
+    b, _ = archive_records(os.path.join(d1, last), idm, jdm)
+    assert a.shape == b.shape and np.array_equal(np.isnan(a), np.isnan(b))
+    out = []
+    for r in range(a.shape[0]):
+        ok = ~np.isnan(a[r])
+        if ok.any():
+            scale = max(float(np.max(np.abs(a[r][ok]))), 1e-30)
+            out.append((float(np.max(np.abs(a[r][ok] - b[r][ok]))) / scale, names[r]))
+    return sorted(out, reverse=True)
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(DROPIN),
                     reason="oracle/_ref/model/hycom_hip not built (make -C oracle/ref model)")
-def test_mixed_dropin_stays_near_the_stock_model(tmp_path):
-    """HYCOM_HYBGEN_PRECISION=mixed: the drop-in runs the mixed form in the
-    model.  Its interfaces are bitwise the FP64 ones per call and its T, S,
-    tracers and velocities within ~1e-6 of them, so after a day the model
-    state stays near the stock one: every archived field within a small
-    fraction of its range."""
+def test_mixed_dropin_stays_within_the_models_own_sensitivity(tmp_path):
+    """HYCOM_HYBGEN_PRECISION=mixed runs the mixed form in the model.  The
+    regrid's decisions are discontinuous in T and S, so after a day any
+    tiny difference has grown wherever a decision flipped.  The yardstick
+    is the stock model started from a T climatology one float32 ulp away
+    (model_case tnudge): the mixed run may differ from the stock run by no
+    more than that run does, per archived record, within a factor 2."""
     import numpy as np
-    print("\n[case] stock model against the drop-in in mixed precision, PPM, 1 day", flush=True)
+    print("\n[case] stock, stock from T one ulp away, drop-in in mixed precision; PPM, 1 day",
+          flush=True)
     kw = dict(hybmap=2, ntracr=1, trcflg=3, days=1.0)
     runs = {}
-    for name, exe, env in (("stock", STOCK, {}),
-                           ("mixed", DROPIN, {"HYCOM_HYBGEN_PRECISION": "mixed"})):
+    for name, exe, env, extra in (("stock", STOCK, {}, {}),
+                                  ("nudged", STOCK, {}, {"tnudge": 1}),
+                                  ("mixed", DROPIN, {"HYCOM_HYBGEN_PRECISION": "mixed"}, {})):
         d = os.path.join(str(tmp_path), name)
-        model_case.write_case(d, **kw)
+        model_case.write_case(d, **kw, **extra)
         with open(os.path.join(d, "run.log"), "w") as log:
             rc = subprocess.run([exe], cwd=d, stdout=log, stderr=subprocess.STDOUT,
                                 env=dict(os.environ, **env), timeout=90).returncode
         text = open(os.path.join(d, "run.log")).read()
         assert rc == 0 and "(normal)" in text, text[-1500:]
         runs[name] = d
-    last = sorted(f for f in os.listdir(runs["stock"]) if f.startswith("archv.") and f.endswith(".a"))[-1]
-    a, names = archive_records(os.path.join(runs["stock"], last), 40, 30)
-    b, _ = archive_records(os.path.join(runs["mixed"], last), 40, 30)
-    assert a.shape == b.shape and np.array_equal(np.isnan(a), np.isnan(b))
-    worst = []
-    for r in range(a.shape[0]):
-        x, y = a[r], b[r]
-        ok = ~np.isnan(x)
-        if not ok.any():
-            continue
-        scale = max(np.nanmax(np.abs(x)), 1e-30)
-        worst.append((float(np.max(np.abs(x[ok] - y[ok]))) / scale, names[r]))
-    worst.sort(reverse=True)
-    print(f"[mixed] largest archived differences over each field's magnitude: {worst[:6]}", flush=True)
-    assert worst[0][0] < 1e-3, worst[:6]
-    assert any(w > 0.0 for w, _ in worst)  # the mixed form did run
+    mixed = record_spread(runs["stock"], runs["mixed"], 40, 30)
+    nudged = dict((n, w) for w, n in record_spread(runs["stock"], runs["nudged"], 40, 30))
+    print(f"[mixed] largest differences over each record's magnitude: {mixed[:5]}", flush=True)
+    print(f"[nudge] the same for the stock run from T one ulp away: "
+          f"{sorted(((w, n) for n, w in nudged.items()), reverse=True)[:5]}", flush=True)
+    assert mixed[0][0] > 0.0  # the mixed form did run
+    top = max(nudged.values())
+    assert mixed[0][0] <= 2.0 * top, (mixed[:5], top)
+    p90m = float(np.percentile([w for w, _ in mixed], 90))
+    p90n = float(np.percentile(list(nudged.values()), 90))
+    assert p90m <= 2.0 * p90n + 1e-9, (p90m, p90n)
