@@ -31,7 +31,7 @@ MODEL = os.path.join(ROOT, "oracle", "_ref", "model")
 STOCK = os.path.join(MODEL, "hycom")
 DROPIN = os.path.join(MODEL, "hycom_hip")
 DENFIX = os.path.join(MODEL, "denfix")  # the three builds with HYBGEN_DENFIX
-STEPS_PER_DAY = 96  # baclin 900 s
+STEPS_PER_DAY = 96  # baclin 900 s, model_case's default
 
 needs_stock = pytest.mark.skipif(not os.path.exists(STOCK),
                                  reason="oracle/_ref/model/hycom not built (make -C oracle/ref model)")
@@ -85,7 +85,7 @@ def first_difference(base):
     return f"first difference after [{phase[-1] if phase else '?'}]:\n" + "\n".join(lines)
 
 
-def assert_identical(rcs, base, mlog, slog, days):
+def assert_identical(rcs, base, mlog, slog, days, baclin=900.0):
     diff = first_difference(base)
     assert diff is None, diff
     assert rcs == [0, 0], f"exit codes {rcs}\n{mlog[-1500:]}\n{slog[-1500:]}"
@@ -93,7 +93,7 @@ def assert_identical(rcs, base, mlog, slog, days):
     # inicon's two calls (inicon.F90:544-560), then one per time step
     assert ok_steps(base, "inicn1") == [0] and ok_steps(base, "inicn2") == [0]
     steps = ok_steps(base, "hybgen")
-    n = int(round(days * STEPS_PER_DAY))
+    n = int(round(days * 86400.0 / baclin))
     assert len(steps) == n and steps[-1] - steps[0] == n - 1, steps[:5]
     # the comparisons read every field: u, v, dp, T, S, th3d of each layer
     assert len(re.findall(r"^reading for comparison: ", base, re.M)) > 100 * n
@@ -157,17 +157,26 @@ def case_kw(kw):
     return {k: v for k, v in kw.items() if k != "denfix"}
 
 
+# on the GPU only (the oracle would take minutes): a domain several waves
+# wide, 41 layers, two tracers, 450 s steps at 0.1 degree
+GPU_CASES = CASES + [
+    ("weno_160x120x41_tracers", dict(hybmap=3, kdm=41, idm=160, jdm=120, res=0.1, baclin=450.0,
+                                     batrop=15.0, ntracr=2, trcflg=23, days=0.25)),
+]
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(DROPIN),
                     reason="oracle/_ref/model/hycom_hip not built (make -C oracle/ref model)")
-@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("name,kw", GPU_CASES, ids=[c[0] for c in GPU_CASES])
 def test_stock_against_dropin(tmp_path, name, kw):
-    days = kw.get("days", 1.0)
+    days, baclin = kw.get("days", 1.0), kw.get("baclin", 900.0)
     print(f"\n[case] stock master (CPU) against the drop-in slave (GPU hybgen), {name}, "
-          f"{int(days * STEPS_PER_DAY)} steps", flush=True)
+          f"{int(round(days * 86400.0 / baclin))} steps", flush=True)
     stock, dropin = builds(kw, "hycom_hip")
-    rcs, base, mlog, slog = run_pair(str(tmp_path), stock, dropin, dict(case_kw(kw), days=days))
-    assert_identical(rcs, base, mlog, slog, days)
+    rcs, base, mlog, slog = run_pair(str(tmp_path), stock, dropin, dict(case_kw(kw), days=days),
+                                     timeout=250)
+    assert_identical(rcs, base, mlog, slog, days, baclin)
     check_tracers_and_archives(str(tmp_path), kw)
 
 
