@@ -556,7 +556,9 @@ HYB_HDI void regrid_stream(const KParams& P, const View& V, const Scratch& W, in
       // rounding, as a start from a climatology gives) makes the second
       // true at kk-1 and the first true at kk; both then take the
       // thickness weighting (min < 0.2*max), so th3d(kk+1) stays unread,
-      // and both move p(kk+1).
+      // and both move p(kk+1).  The second test stays off at kk: it can
+      // pass there only with two layers of negative thickness, and would
+      // move p(kk+2), an interface the column does not have.
       if (dp_cen < P.hybthk * dmin(dp_top, dp_bot)) {
         // 1131-1159 thick-thin-thick
         const double thp = thw[2], tkp = tw[2];
