@@ -143,6 +143,12 @@ CASES = [
     ("plm_tracer_diag", dict(hybmap=1, ntracr=1, trcflg=3, diag=(803, 801))),
     ("ppm_dx0k_noiso_norelax", dict(hybmap=2, dx00=(30.0, 300.0, 1.1), hybiso=0.0,
                                     hybrlx=1.0)),
+    # combinations
+    ("weno_periodic_ra_tracers_denfix", dict(hybmap=3, periodic=True, hybraf=True, ntracr=2,
+                                             trcflg=23, denfix=True)),
+    ("plm_sigma_dx0k_ra_diag", dict(hybmap=1, nsigma=4, hybthk=0.9, shelf=8.0,
+                                    dx00=(30.0, 300.0, 1.1), hybraf=True, ntracr=1, trcflg=2,
+                                    diag=(802,))),
 ]
 
 
