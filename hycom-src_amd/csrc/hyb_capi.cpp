@@ -328,6 +328,15 @@ struct Stager {
     fill = 0;
     return hipSuccess;
   }
+  // after a failed call, once its stream has drained: nothing is in flight
+  // and the downloads queued so far are not copied out
+  void drop() {
+    for (int b = 0; b < 2; b++) {
+      armed[b] = false;
+      pending[b].clear();
+    }
+    fill = 0;
+  }
 };
 
 Stager g_stager;  // under g_mirror_mu; its pinned buffers live until the process ends
@@ -1130,72 +1139,80 @@ int hycom_hybgen(const hyb_params* P, const hyb_fields* F, int m, int n, int raf
   }
   hipStream_t st = M.stream;
   Stager& X = g_stager;
-  HIPCHK(X.begin(st));
   int64_t up = 0, down = 0;
-  auto h2d = [&](void* d, const void* h, size_t bytes) {
-    up += (int64_t)bytes;
-    return X.up(d, h, bytes);
-  };
-  auto d2h = [&](void* h, const void* d, size_t bytes) {
-    down += (int64_t)bytes;
-    return X.down(h, d, bytes);
-  };
-  // the fields of time level l (1-based) that the call reads and writes
-  const size_t b3 = n3 * sizeof(double);
-  auto level = [&](int l, bool to_device) -> hipError_t {
-    const size_t o = n3 * (size_t)(l - 1);
-    double* dev[] = {M.dp, M.dpu, M.dpv, M.temp, M.saln, M.th3d, M.u, M.v};
-    double* host[] = {F->dp, F->dpu, F->dpv, F->temp, F->saln, F->th3d, F->u, F->v};
-    hipError_t e = hipSuccess;
-    for (int q = 0; q < 8 && e == hipSuccess; q++)
-      e = to_device ? h2d(dev[q] + o, host[q] + o, b3) : d2h(host[q] + o, dev[q] + o, b3);
-    for (int t = 0; t < s.ntr && e == hipSuccess; t++) {
-      const size_t ot = n4 * (size_t)t + o;
-      e = to_device ? h2d(M.tracer + ot, F->tracer + ot, b3)
-                    : d2h(F->tracer + ot, M.tracer + ot, b3);
+  auto call = [&]() -> int {
+    HIPCHK(X.begin(st));
+    auto h2d = [&](void* d, const void* h, size_t bytes) {
+      up += (int64_t)bytes;
+      return X.up(d, h, bytes);
+    };
+    auto d2h = [&](void* h, const void* d, size_t bytes) {
+      down += (int64_t)bytes;
+      return X.down(h, d, bytes);
+    };
+    // the fields of time level l (1-based) that the call reads and writes
+    const size_t b3 = n3 * sizeof(double);
+    auto level = [&](int l, bool to_device) -> hipError_t {
+      const size_t o = n3 * (size_t)(l - 1);
+      double* dev[] = {M.dp, M.dpu, M.dpv, M.temp, M.saln, M.th3d, M.u, M.v};
+      double* host[] = {F->dp, F->dpu, F->dpv, F->temp, F->saln, F->th3d, F->u, F->v};
+      hipError_t e = hipSuccess;
+      for (int q = 0; q < 8 && e == hipSuccess; q++)
+        e = to_device ? h2d(dev[q] + o, host[q] + o, b3) : d2h(host[q] + o, dev[q] + o, b3);
+      for (int t = 0; t < s.ntr && e == hipSuccess; t++) {
+        const size_t ot = n4 * (size_t)t + o;
+        e = to_device ? h2d(M.tracer + ot, F->tracer + ot, b3)
+                      : d2h(F->tracer + ot, M.tracer + ot, b3);
+      }
+      if (to_device && e == hipSuccess) {
+        const size_t ob = n2 * (size_t)(l - 1);
+        e = h2d(M.ubavg + ob, F->ubavg + ob, n2 * sizeof(double));
+        if (e == hipSuccess) e = h2d(M.vbavg + ob, F->vbavg + ob, n2 * sizeof(double));
+      }
+      return e;
+    };
+    HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
+    HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
+    HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
+    HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
+    HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
+    HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
+    HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
+    HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
+    HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
+    HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
+    HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
+    HIPCHK(level(n, true));
+    if (raflag) {
+      HIPCHK(level(m, true));
+      HIPCHK(h2d(M.dpo, F->dpo, n4 * sizeof(double)));
     }
-    if (to_device && e == hipSuccess) {
-      const size_t ob = n2 * (size_t)(l - 1);
-      e = h2d(M.ubavg + ob, F->ubavg + ob, n2 * sizeof(double));
-      if (e == hipSuccess) e = h2d(M.vbavg + ob, F->vbavg + ob, n2 * sizeof(double));
-    }
-    return e;
+    hyb_fields D;
+    D.dp = M.dp; D.dpo = raflag ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
+    D.temp = M.temp; D.saln = M.saln; D.th3d = M.th3d; D.u = M.u; D.v = M.v;
+    D.tracer = s.ntr > 0 ? M.tracer : nullptr;
+    D.p = M.p; D.pu = M.pu; D.pv = M.pv; D.theta = M.theta;
+    D.depths = M.depths; D.topiso = M.topiso; D.depthu = M.depthu; D.depthv = M.depthv;
+    D.ubavg = M.ubavg; D.vbavg = M.vbavg; D.ip = M.ip; D.iu = M.iu; D.iv = M.iv;
+    const int drc = hycom_hybgen_device(P, &D, m, n, raflag, st);
+    if (drc) return drc;
+    HIPCHK(level(n, false));
+    if (raflag) HIPCHK(level(m, false));
+    HIPCHK(d2h(F->p, M.p, n3p * sizeof(double)));
+    HIPCHK(d2h(F->pu, M.pu, n3p * sizeof(double)));
+    HIPCHK(d2h(F->pv, M.pv, n3p * sizeof(double)));
+    HIPCHK(X.finish());
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
   };
-  HIPCHK(h2d(M.theta, F->theta, n3 * sizeof(double)));
-  HIPCHK(h2d(M.depths, F->depths, n2 * sizeof(double)));
-  HIPCHK(h2d(M.topiso, F->topiso, n2 * sizeof(double)));
-  HIPCHK(h2d(M.depthu, F->depthu, n2 * sizeof(double)));
-  HIPCHK(h2d(M.depthv, F->depthv, n2 * sizeof(double)));
-  HIPCHK(h2d(M.ip, F->ip, n2 * sizeof(int32_t)));
-  HIPCHK(h2d(M.iu, F->iu, n2 * sizeof(int32_t)));
-  HIPCHK(h2d(M.iv, F->iv, n2 * sizeof(int32_t)));
-  HIPCHK(h2d(M.p, F->p, n3p * sizeof(double)));
-  HIPCHK(h2d(M.pu, F->pu, n3p * sizeof(double)));
-  HIPCHK(h2d(M.pv, F->pv, n3p * sizeof(double)));
-  HIPCHK(level(n, true));
-  if (raflag) {
-    HIPCHK(level(m, true));
-    HIPCHK(h2d(M.dpo, F->dpo, n4 * sizeof(double)));
-  }
-  hyb_fields D;
-  D.dp = M.dp; D.dpo = raflag ? M.dpo : nullptr; D.dpu = M.dpu; D.dpv = M.dpv;
-  D.temp = M.temp; D.saln = M.saln; D.th3d = M.th3d; D.u = M.u; D.v = M.v;
-  D.tracer = s.ntr > 0 ? M.tracer : nullptr;
-  D.p = M.p; D.pu = M.pu; D.pv = M.pv; D.theta = M.theta;
-  D.depths = M.depths; D.topiso = M.topiso; D.depthu = M.depthu; D.depthv = M.depthv;
-  D.ubavg = M.ubavg; D.vbavg = M.vbavg; D.ip = M.ip; D.iu = M.iu; D.iv = M.iv;
-  rc = hycom_hybgen_device(P, &D, m, n, raflag, st);
+  rc = call();
   if (rc) {
+    // copies queued before the failure may still read or fill the staging
+    // buffers: let the stream drain before the next call reuses them
     (void)hipStreamSynchronize(st);
+    X.drop();
     return rc;
   }
-  HIPCHK(level(n, false));
-  if (raflag) HIPCHK(level(m, false));
-  HIPCHK(d2h(F->p, M.p, n3p * sizeof(double)));
-  HIPCHK(d2h(F->pu, M.pu, n3p * sizeof(double)));
-  HIPCHK(d2h(F->pv, M.pv, n3p * sizeof(double)));
-  HIPCHK(X.finish());
-  HIPCHK(hipStreamSynchronize(st));
   M.up = up;
   M.down = down;
   return 0;
