@@ -10,6 +10,11 @@ Pinning, bottom up:
   elementwise blend and interfaces.  mod_stokes.F90 keeps sdzi and usdz
   private, so the reference's stokes_vertical_j itself cannot be driven
   from outside; its six lines around the two calls are these numpy lines;
+* the oracle's whole call against a closed form that restates none of
+  those lines: with snapshots that zigzag in depth every PLM slope is zero
+  and each model layer is the exact mean of the blended profile over it;
+  the test shows that an interface off by one, swapped weights and the
+  other time level's dp each fail the same check;
 * the library's column code (walk_core with ki != ko, compiled for the
   host) against the oracle, bitwise, on domains whose fixed layers end
   above, at and below the bottom.
@@ -125,6 +130,80 @@ def test_library_column_code_matches_oracle(kdm, nsdzi, depth_max, n):
     assert np.array_equal(a.vsdp, b.vsdp)
     land = sd.dom.a["ip"] == 0
     assert np.all(b.usdp[land] == -7.0) and np.all(b.vsdp[land] == 7.0)
+
+
+def _overlap_means(s, zi, zo):
+    """Thickness-weighted mean of the piecewise-constant profile s (value
+    s[l] on [zi[l], zi[l+1]]) over each layer [zo[k], zo[k+1]]; NaN for
+    empty layers."""
+    lo = np.maximum(zo[:-1, None], zi[None, :-1])
+    hi = np.minimum(zo[1:, None], zi[None, 1:])
+    w = np.maximum(hi - lo, 0.0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return (w @ s) / (zo[1:] - zo[:-1])
+
+
+@pytest.mark.parametrize("ws0,n", [(0.3, 2), (1.0, 1), (0.0, 2), (0.75, 1)])
+def test_oracle_call_is_the_layer_mean_of_a_zigzag_profile(ws0, n):
+    """A closed form of stokes_vertical_j (mod_stokes.F90:676-697) that needs
+    neither the reference nor its code lines.  When both snapshots zigzag in
+    depth (every fixed layer a strict extremum of its neighbours, after the
+    blend too) the PLM limiter (hybgen.F90:2423-2436) gives every layer a
+    zero slope, so each model layer's value is the exact mean, over the
+    layer, of the time-interpolated profile: fixed layer l spans
+    [sdzi(l-1), sdzi(l)] (sdzi(0) = 0), model layer k spans the running sum
+    of dp(k, n), both cut at the sea floor p(kk+1).  Swapped blend weights
+    or snapshots, u taken for v, an interface off by one or dp of the
+    other time level each fail it."""
+    sd = _domain(32, 16, seed=int(ws0 * 8) + n, depth_max=150.0, ws0=ws0, n=n)
+    P, a, nb, kk = sd.P, sd.dom.a, sd.P.nbdy, sd.P.kdm
+    rng = np.random.default_rng(17 + n)
+    NI, NJ, _, nz = sd.usdz.shape
+    sign = (-1.0) ** np.arange(nz)
+    for arr in (sd.usdz, sd.vsdz):
+        base = rng.uniform(-0.5, 0.5, (NI, NJ, 2, 1))
+        amp = rng.uniform(0.01, 0.2, (NI, NJ, 2, 1)) * rng.uniform(0.5, 1.0, (NI, NJ, 2, nz))
+        arr[...] = base + amp * sign
+    # the other time level holds the same thicknesses upside down
+    a["dp"][:, :, :, 2 - n] = a["dp"][:, :, ::-1, n - 1]
+    ol.run_oracle_stokes(sd)
+    assert sd.sdzi[-1] > a["p"][:, :, kk].max()       # the fixed layers reach every floor
+    checked, caught = 0, [0, 0, 0]
+    for j in range(1, P.jj + 1):
+        for i in range(1, P.ii + 1):
+            I, J = i - 1 + nb, j - 1 + nb
+            if a["ip"][I, J] == 0:
+                continue
+            pbot = a["p"][I, J, kk]
+            zi = np.concatenate([[0.0], np.minimum(sd.sdzi, pbot)])
+            zo = np.minimum(np.concatenate([[0.0], np.cumsum(a["dp"][I, J, :, n - 1])]), pbot)
+            thick = np.diff(zo) > 1.0e-3 * ONEM
+            # the same closed form with a planted mistake, to show it discriminates
+            zo_lvl = np.minimum(np.concatenate([[0.0], np.cumsum(a["dp"][I, J, :, 2 - n])]), pbot)
+            zi_off = np.concatenate([[0.0], np.minimum(np.append(sd.sdzi[1:], sd.sdzi[-1]), pbot)])
+            for snap, out in ((sd.usdz, sd.usdp), (sd.vsdz, sd.vsdp)):
+                s = P.ws0 * snap[I, J, 0, :] + P.ws1 * snap[I, J, 1, :]
+                want = _overlap_means(s, zi, zo)
+                got = out[I, J, :]
+                scale = np.abs(s).max()
+                assert np.all(np.abs(got[thick] - want[thick]) <= 1e-12 * scale), (i, j)
+                # thin layers lie within the range of the profile
+                assert np.all(got >= s.min() - 1e-12 * scale) and np.all(got <= s.max() + 1e-12 * scale)
+                wrong = [_overlap_means(s, zi_off, zo)]
+                if ws0 != 0.5:
+                    wrong.append(_overlap_means(P.ws1 * snap[I, J, 0, :] + P.ws0 * snap[I, J, 1, :],
+                                                zi, zo))
+                for q, w in enumerate(wrong):
+                    caught[q] += bool(np.any(np.abs(got[thick] - w[thick]) > 1e-9 * scale))
+                t2 = np.diff(zo_lvl) > 1.0e-3 * ONEM
+                w = _overlap_means(s, zi, zo_lvl)
+                caught[2] += bool(np.any(np.abs(got[t2] - w[t2]) > 1e-9 * scale)
+                                  or not np.array_equal(t2, thick))
+            checked += 1
+    assert checked > 100
+    # every planted mistake is seen in most columns (both fields counted)
+    assert caught[0] > checked and caught[2] > checked
+    assert ws0 == 0.5 or caught[1] > checked
 
 
 def test_library_column_code_short_model_column():
