@@ -1660,6 +1660,11 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
       break;
     }
     // ------------------------------------------ slide to layer l+1
+    // The slide's look-ahead loads issue at raised wave priority, ahead of
+    // the arithmetic of the other waves on the SIMD (values unaffected).
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_setprio(2);
+#endif
     const double top = pv4;
     pv0 = pv1;
     pv1 = pv2;
@@ -1697,6 +1702,9 @@ HYB_HDI void walk_core(int K, double zx, const PI& pi, PO& po, SINK& sink, const
         sq[f][PQ - 1] = (f < nf && xl <= K) ? raw(f, xl) : RawT{};
       }
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_setprio(0);
+#endif
     xl++;
   }
   double lo[NF], hi[NF];
